@@ -1,0 +1,372 @@
+/* infernal_gpu.h -- C ABI of the MI355X QDB scanning CYK/Inside library.
+ *
+ * The five scanner entry points keep the argument lists of Infernal 1.1.5
+ * (src/infernal.h:2710-2730) so that src/cm_pipeline.c:pli_dispatch_cm_search
+ * (:4187-4210) and src/cmcalibrate.c:process_search_workunit (:2337-2358) can
+ * call them unchanged.  The structures below are this library's own mirrors of
+ * the fields those scanners read from Infernal's CM_t, CM_SCAN_MX,
+ * CM_TR_SCAN_MX and CM_TOPHITS; they carry no Easel types.
+ *
+ * All scanners run on the GPU (gfx950).  There is no CPU fallback: when no
+ * usable HIP device is present a scanner returns eslENODEVICE with a message in
+ * errbuf.
+ */
+#ifndef INFERNAL_GPU_H
+#define INFERNAL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Status codes.  Values follow Easel's easel.h (not vendored with the
+ * reference; numbers taken from Easel's published header, to verify against a
+ * real Easel checkout).
+ */
+#define eslOK           0
+#define eslFAIL         1
+#define eslEOF          3
+#define eslEMEM         5
+#define eslENOTFOUND    6
+#define eslEFORMAT      7
+#define eslEINCOMPAT   10
+#define eslEINVAL      11
+#define eslESYS        12
+#define eslERANGE      16
+#define eslENODEVICE  100   /* this library only: no usable GPU */
+#define eslERRBUFSIZE 128
+
+typedef uint8_t ESL_DSQ;
+
+/* RNA alphabet, Easel layout "ACGU-RYMKSWHBVDN*~": K=4 canonical residues,
+ * code 4 gap, 5..15 degenerate, 16 '*' (no residue), 17 '~' (missing). */
+#define INFX_K   4
+#define INFX_KP 18
+
+/* Score constants (src/infernal.h:148-177). */
+#define IMPOSSIBLE   -1e36
+#define INFTY        987654321
+#define INTSCALE     1000.0f
+#define LOGSUM_TBL   23000
+#define NOT_IMPOSSIBLE(x) ((x) > -9.999e35)
+#define MAXCONNECT   6
+
+/* State types (src/infernal.h:193-202). */
+#define D_st  0
+#define MP_st 1
+#define ML_st 2
+#define MR_st 3
+#define IL_st 4
+#define IR_st 5
+#define S_st  6
+#define E_st  7
+#define B_st  8
+#define EL_st 9
+
+/* Node types (src/infernal.h:208-216). */
+#define DUMMY_nd -1
+#define BIF_nd    0
+#define MATP_nd   1
+#define MATL_nd   2
+#define MATR_nd   3
+#define BEGL_nd   4
+#define BEGR_nd   5
+#define ROOT_nd   6
+#define END_nd    7
+
+/* Unique state ids (src/infernal.h:221-243). */
+#define ROOT_S  0
+#define ROOT_IL 1
+#define ROOT_IR 2
+#define BEGL_S  3
+#define BEGR_S  4
+#define BEGR_IL 5
+#define MATP_MP 6
+#define MATP_ML 7
+#define MATP_MR 8
+#define MATP_D  9
+#define MATP_IL 10
+#define MATP_IR 11
+#define MATL_ML 12
+#define MATL_D  13
+#define MATL_IL 14
+#define MATR_MR 15
+#define MATR_D  16
+#define MATR_IR 17
+#define END_E   18
+#define BIF_B   19
+#define END_EL  20
+
+/* Emit modes (src/infernal.h:180-185). */
+#define EMITLEFT  0
+#define EMITRIGHT 1
+#define EMITPAIR  2
+#define EMITNONE  3
+
+/* cm->flags (src/infernal.h:1927-1939, subset). */
+#define CMH_BITS         (1<<0)
+#define CMH_LOCAL_BEGIN  (1<<10)
+#define CMH_LOCAL_END    (1<<11)
+#define CM_IS_CONFIGURED (1<<23)
+
+/* cm->config_opts (src/infernal.h:1953-1962, subset). */
+#define CM_CONFIG_LOCAL     (1<<0)
+#define CM_CONFIG_SCANMX    (1<<6)
+#define CM_CONFIG_TRSCANMX  (1<<7)
+#define CM_CONFIG_TRUNC     (1<<5)
+
+/* cm->search_opts (src/infernal.h:1990-2001, subset). */
+#define CM_SEARCH_INSIDE       (1<<5)
+#define CM_SEARCH_CMNOTGREEDY  (1<<8)
+#define CM_SEARCH_NULL3        (1<<11)
+
+/* QDB indices (src/infernal.h:1607-1609). */
+#define SMX_NOQDB       0
+#define SMX_QDB1_TIGHT  1
+#define SMX_QDB2_LOOSE  2
+#define NSMX_QDB_IDX    3
+
+/* Truncation modes and penalties (src/infernal.h:341-345, :1723-1727). */
+#define TRMODE_T        0
+#define TRMODE_R        1
+#define TRMODE_L        2
+#define TRMODE_J        3
+#define TRMODE_UNKNOWN  4
+#define TRPENALTY_5P_AND_3P 0
+#define TRPENALTY_5P_ONLY   1
+#define TRPENALTY_3P_ONLY   2
+#define NTRPENALTY          3
+#define TRPENALTY_NONE     -1
+
+/* Pipeline pass indices (src/infernal.h:2097-2103). */
+#define PLI_PASS_CM_SUMMED       0
+#define PLI_PASS_STD_ANY         1
+#define PLI_PASS_5P_ONLY_FORCE   2
+#define PLI_PASS_3P_ONLY_FORCE   3
+#define PLI_PASS_5P_AND_3P_FORCE 4
+#define PLI_PASS_5P_AND_3P_ANY   5
+#define PLI_PASS_HMM_ONLY_ANY    6
+
+/* Hit flags (src/infernal.h:2406-2410, subset). */
+#define CM_HIT_FLAGS_DEFAULT         0
+#define CM_HIT_IS_INCLUDED           (1<<0)
+#define CM_HIT_IS_REPORTED           (1<<1)
+#define CM_HIT_IS_REMOVED_DUPLICATE  (1<<2)
+
+/* ---------------------------------------------------------------------------
+ * Model.  Field names follow Infernal's CM_t (src/infernal.h:1759-1924).
+ * Two-dimensional arrays are row pointers into one contiguous block, as in
+ * CreateCMBody (src/cm.c:179).
+ */
+typedef struct cm_qdbinfo_s {
+  int    *dmin1, *dmax1;   /* [0..M-1] tight bands (beta1) */
+  int    *dmin2, *dmax2;   /* [0..M-1] loose bands (beta2) */
+  double  beta1, beta2;
+} CM_QDBINFO;
+
+typedef struct cm_emitmap_s {
+  int  clen;
+  int *lpos, *rpos, *epos; /* [0..nodes-1] */
+} CMEmitMap_t;
+
+typedef struct cm_tr_penalties_s {
+  int     M;
+  int     ignored_inserts;
+  float **g_ptyAA;         /* [NTRPENALTY][0..M-1] global truncated-begin scores */
+  float **l_ptyAA;         /* [NTRPENALTY][0..M-1] local truncated-begin scores */
+  int   **ig_ptyAA;        /* integer (x1000) versions */
+  int   **il_ptyAA;
+} CM_TR_PENALTIES;
+
+struct cm_scan_mx_s;
+struct cm_tr_scan_mx_s;
+
+typedef struct cm_s {
+  char   *name;
+  int     M, nodes, clen, W;
+  float   pbegin, pend;
+  double  beta_W;
+  float   null3_omega;
+  float  *null;            /* [K] background */
+  char   *sttype;          /* [M] */
+  char   *stid;            /* [M] */
+  int    *ndidx;           /* [M] */
+  int    *cfirst, *cnum;   /* [M] */
+  int    *plast, *pnum;    /* [M] */
+  int    *nodemap;         /* [nodes] */
+  char   *ndtype;          /* [nodes] */
+  float **t, **e;          /* [M][MAXCONNECT], [M][K*K] probabilities */
+  float  *begin, *end;     /* [M] local begin/end probabilities */
+  float **tsc, **esc;      /* log-odds bits */
+  float **lmesc, **rmesc;  /* [M][Kp] marginal emission scores (truncation) */
+  float  *beginsc, *endsc;
+  int   **itsc, **iesc, **ilmesc, **irmesc;
+  int    *ibeginsc, *iendsc;
+  float **oesc;            /* [M] -> Kp (singlets) or Kp*Kp (pairs); NULL for non-emitters */
+  int   **ioesc;
+  float   el_selfsc;
+  int     iel_selfsc;
+  float  *root_trans;      /* saved t[0][] before localization */
+  int     flags, config_opts, search_opts;
+  CM_QDBINFO      *qdbinfo;
+  CMEmitMap_t     *emap;
+  CM_TR_PENALTIES *trp;
+  struct cm_scan_mx_s    *smx;
+  struct cm_tr_scan_mx_s *trsmx;
+} CM_t;
+
+/* Scan matrix.  Mirrors CM_SCAN_MX (src/infernal.h:1600-1650): the band
+ * tables dnAAA/dxAAA are filled exactly as cm_scan_mx_Create
+ * (src/cm_mx.c:5725-5739) does; the DP matrices themselves live in device
+ * memory owned by `dev`. */
+typedef struct cm_scan_mx_s {
+  int    M, W;
+  int ***dnAAA, ***dxAAA;  /* [qdbidx][1..W][0..M-1] */
+  int    floats_valid, ints_valid;
+  float *bestsc;           /* [0..W] */
+  int   *bestr;            /* [0..W] */
+  void  *dev;              /* device resources, owned by this library */
+} CM_SCAN_MX;
+
+/* Truncated scan matrix.  Mirrors CM_TR_SCAN_MX; bands as cm_tr_scan_mx_Create
+ * (src/cm_mx.c:6428-6441): dn == 1 for every state, only dmax is used. */
+typedef struct cm_tr_scan_mx_s {
+  int    M, W;
+  int ***dnAAA, ***dxAAA;
+  int    floats_valid, ints_valid;
+  float *bestsc;
+  int   *bestr;
+  char  *bestmode;
+  void  *dev;
+} CM_TR_SCAN_MX;
+
+/* Hits.  Mirrors CM_HIT / CM_TOPHITS (src/infernal.h:2424-2480), fields the
+ * scanners set. */
+typedef struct cm_hit_s {
+  int64_t start, stop;
+  int64_t srcL;
+  int64_t hit_idx;
+  int64_t cm_idx, seq_idx;
+  int     pass_idx;
+  int     in_rc;
+  int     root;
+  char    mode;
+  float   score;
+  int     hmmonly;
+  int     flags;
+} CM_HIT;
+
+typedef struct cm_tophits_s {
+  CM_HIT **hit;            /* sorted view */
+  CM_HIT  *unsrt;          /* storage, in creation order */
+  uint64_t Nalloc;
+  uint64_t N;
+  int      is_sorted_for_overlap_removal;
+} CM_TOPHITS;
+
+/* ---------------------------------------------------------------------------
+ * Model input and configuration (host).
+ */
+
+/* Read the first model of an INFERNAL1/a save file.  Restates
+ * read_asc_1p1_cm (src/cm_file.c:1539-2118) + CMRenormalize (src/cm.c:344) +
+ * CreateEmitMap (src/display.c:1110).  The filter HMM that follows the model
+ * is skipped. */
+int  infx_cm_Read(const char *path, CM_t **ret_cm, char *errbuf);
+/* Same, from an in-memory text buffer. */
+int  infx_cm_ReadText(const char *text, CM_t **ret_cm, char *errbuf);
+void FreeCM(CM_t *cm);
+
+/* cm_Configure (src/cm_modelconfig.c:98, :139-373) for the options the
+ * scanners need: CM_CONFIG_LOCAL (cm_localize :467-548), CM_CONFIG_TRUNC
+ * (cm_tr_penalties_Create src/cm_trunc.c:51), CM_CONFIG_SCANMX and
+ * CM_CONFIG_TRSCANMX (scan matrices), then CMLogoddsify (src/cm.c:784).
+ * W_from_cmdline is -1 to keep the file's W.  QDBs are taken from the file. */
+int  cm_Configure(CM_t *cm, char *errbuf, int W_from_cmdline);
+
+/* cm_scan_mx_Create (src/cm_mx.c:5686) / cm_tr_scan_mx_Create (:6391). */
+int  cm_scan_mx_Create(CM_t *cm, char *errbuf, int do_float, int do_int, CM_SCAN_MX **ret_smx);
+void cm_scan_mx_Destroy(CM_t *cm, CM_SCAN_MX *smx);
+int  cm_tr_scan_mx_Create(CM_t *cm, char *errbuf, int do_float, int do_int, CM_TR_SCAN_MX **ret_trsmx);
+void cm_tr_scan_mx_Destroy(CM_t *cm, CM_TR_SCAN_MX *trsmx);
+
+/* Log-sum tables (src/logsum.c:74-169). */
+void  FLogsumInit(void);
+float FLogsum(float s1, float s2);
+void  init_ilogsum(void);
+int   ILogsum(int s1, int s2);
+
+/* Digitize an RNA/DNA text sequence into dsq[0..L+1] with sentinels (Easel
+ * esl_abc_Digitize conventions: case-insensitive, T->U, X->N, '.'/'_'->gap).
+ * Returns eslEINVAL on an unknown character. */
+int  infx_abc_Digitize(const char *seq, int64_t L, ESL_DSQ *dsq);
+
+/* ---------------------------------------------------------------------------
+ * Hit lists (src/cm_tophits.c). */
+CM_TOPHITS *cm_tophits_Create(void);
+void        cm_tophits_Destroy(CM_TOPHITS *th);
+int         cm_tophits_CreateNextHit(CM_TOPHITS *th, CM_HIT **ret_hit);
+int         cm_tophits_SortForOverlapRemoval(CM_TOPHITS *th);
+int         cm_tophits_RemoveOrMarkOverlaps(CM_TOPHITS *th, int do_clans_only, char *errbuf);
+int         cm_tophits_CloneHitMostly(CM_TOPHITS *src_th, int h, CM_TOPHITS *dest_th);
+
+/* ---------------------------------------------------------------------------
+ * The scanners (GPU).  Contracts, return values and outputs as the reference
+ * functions of the same name; hits are appended to <hitlist> after greedy
+ * overlap removal (or gamma traceback when cm->search_opts has
+ * CM_SEARCH_CMNOTGREEDY).  *ret_vsc, when requested, is malloc'ed and owned
+ * by the caller.
+ */
+
+/* src/cm_dpsearch.c:90 FastCYKScan */
+int FastCYKScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc);
+/* src/cm_dpsearch.c:1731 FastFInsideScan */
+int FastFInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                    float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                    int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc);
+/* src/cm_dpsearch.c:1106 FastIInsideScan */
+int FastIInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                    float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                    int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc);
+/* src/cm_dpsearch_trunc.c:79 RefTrCYKScan */
+int RefTrCYKScan(CM_t *cm, char *errbuf, CM_TR_SCAN_MX *trsmx, int qdbidx, int pass_idx, ESL_DSQ *dsq,
+                 int64_t i0, int64_t j0, float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                 int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, char *ret_mode, float *ret_sc);
+/* src/cm_dpsearch_trunc.c:741 RefITrInsideScan */
+int RefITrInsideScan(CM_t *cm, char *errbuf, CM_TR_SCAN_MX *trsmx, int qdbidx, int pass_idx, ESL_DSQ *dsq,
+                     int64_t i0, int64_t j0, float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                     int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, char *ret_mode, float *ret_sc);
+
+/* src/cm_dpsearch_trunc.c:3566 cm_TrFillFromPassIdx, src/cm_trunc.c:464
+ * cm_tr_penalties_IdxForPass, src/cm_pipeline.c:2396
+ * cm_pli_PassAllowsTruncation, src/cm_tophits.c:3048 cm_hit_AllowTruncation */
+int cm_TrFillFromPassIdx(int pass_idx, int *ret_fill_L, int *ret_fill_R, int *ret_fill_T);
+int cm_tr_penalties_IdxForPass(int pass_idx);
+int cm_pli_PassAllowsTruncation(int pass_idx);
+int cm_hit_AllowTruncation(CM_t *cm, int pass_idx, int64_t start, int64_t stop, int64_t i0, int64_t j0, char mode, int b);
+
+/* src/cm_dpsearch.c:3099 cm_CountSearchDPCalcs: DP calcs per residue (the
+ * reference's definition), for forecasts.  Also: this library's cell count,
+ * the number of (v,d) cells inside the band at steady state (jp >= W). */
+int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells_per_res, double *ret_ops_per_res);
+
+/* ---------------------------------------------------------------------------
+ * Library-level controls and introspection (no counterpart in the reference).
+ */
+int  infx_gpu_available(char *errbuf);                 /* 1 if a gfx9 HIP device can be used */
+/* Timing of the last scanner call on this thread: total wall ms around the
+ * call, and the summed device duration of the DP kernel launches in it
+ * (HIP events on the library's stream). */
+void infx_last_call_timing(double *ret_wall_ms, double *ret_kernel_ms, int64_t *ret_cells, int *ret_launches);
+/* Override the work split (0 = automatic): residues per chunk and waves per
+ * block.  Used by the autotuner and tests. */
+void infx_set_launch_params(int chunk_len, int waves_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INFERNAL_GPU_H */

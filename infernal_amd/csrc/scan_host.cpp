@@ -1,0 +1,671 @@
+/* scan_host.cpp -- host side of the GPU scanners: scan matrices (band
+ * tables), the device "scan profile" of a configured CM, the launch, and the
+ * C-ABI entry points FastCYKScan / FastFInsideScan / FastIInsideScan.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cm_internal.h"
+#include "scan_kernels.h"
+#include "scan_host.h"
+
+extern "C" hipError_t infx_launch_std(int semi, int want_vsc, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a);
+
+/* ------------------------------------------------------------------------ */
+/* scan matrices (src/cm_mx.c:5686-5836, :6391-6560)                          */
+
+static int ***alloc_bands(int M, int W)
+{
+  int ***A = (int ***) calloc(NSMX_QDB_IDX, sizeof(int **));
+  for (int n = 0; n < NSMX_QDB_IDX; n++) {
+    A[n] = (int **) calloc(W + 1, sizeof(int *));
+    int *blk = (int *) calloc((size_t) (W + 1) * M, sizeof(int));
+    for (int j = 0; j <= W; j++) A[n][j] = blk + (size_t) j * M;
+    A[n][0] = blk;  /* row 0 is unused (j == 0 is out of bounds), it owns the block */
+  }
+  return A;
+}
+static void free_bands(int ***A)
+{
+  if (!A) return;
+  for (int n = 0; n < NSMX_QDB_IDX; n++) { free(A[n][0]); free(A[n]); }
+  free(A);
+}
+
+extern "C" int cm_scan_mx_Create(CM_t *cm, char *errbuf, int do_float, int do_int, CM_SCAN_MX **ret_smx)
+{
+  *ret_smx = NULL;
+  if (!do_float && !do_int) return infx_fail(eslEINVAL, errbuf, "cm_scan_mx_Create(), do_float and do_int both FALSE.");
+  if (cm->qdbinfo == NULL) return infx_fail(eslEINVAL, errbuf, "cm_scan_mx_Create(), qdbinfo is invalid");
+  CM_SCAN_MX *smx = (CM_SCAN_MX *) calloc(1, sizeof(CM_SCAN_MX));
+  smx->M = cm->M; smx->W = cm->W;
+  smx->dnAAA = alloc_bands(cm->M, cm->W);
+  smx->dxAAA = alloc_bands(cm->M, cm->W);
+  const CM_QDBINFO *q = cm->qdbinfo;
+  for (int j = 1; j <= smx->W; j++)
+    for (int v = 0; v < smx->M; v++) {
+      int mp = (cm->sttype[v] == MP_st);
+      smx->dnAAA[SMX_NOQDB][j][v]      = ESL_MIN(mp ? 2 : 1, smx->W);
+      smx->dnAAA[SMX_QDB1_TIGHT][j][v] = ESL_MIN(mp ? ESL_MAX(q->dmin1[v], 2) : ESL_MAX(q->dmin1[v], 1), smx->W);
+      smx->dnAAA[SMX_QDB2_LOOSE][j][v] = ESL_MIN(mp ? ESL_MAX(q->dmin2[v], 2) : ESL_MAX(q->dmin2[v], 1), smx->W);
+      smx->dxAAA[SMX_NOQDB][j][v]      = j;
+      smx->dxAAA[SMX_QDB1_TIGHT][j][v] = ESL_MIN(j, ESL_MIN(q->dmax1[v], smx->W));
+      smx->dxAAA[SMX_QDB2_LOOSE][j][v] = ESL_MIN(j, ESL_MIN(q->dmax2[v], smx->W));
+    }
+  smx->bestsc = (float *) malloc(sizeof(float) * (smx->W + 1));
+  smx->bestr  = (int *) malloc(sizeof(int) * (smx->W + 1));
+  for (int d = 0; d <= smx->W; d++) { smx->bestsc[d] = IMPOSSIBLE; smx->bestr[d] = 0; }
+  smx->floats_valid = do_float ? 1 : 0;
+  smx->ints_valid   = do_int ? 1 : 0;
+  smx->dev = NULL;
+  *ret_smx = smx;
+  return eslOK;
+}
+
+extern "C" void cm_scan_mx_Destroy(CM_t *cm, CM_SCAN_MX *smx)
+{
+  (void) cm;
+  if (!smx) return;
+  infx_dev_release(smx->dev);
+  free_bands(smx->dnAAA); free_bands(smx->dxAAA);
+  free(smx->bestsc); free(smx->bestr);
+  free(smx);
+}
+
+extern "C" int cm_tr_scan_mx_Create(CM_t *cm, char *errbuf, int do_float, int do_int, CM_TR_SCAN_MX **ret_trsmx)
+{
+  *ret_trsmx = NULL;
+  if (!do_float && !do_int) return infx_fail(eslEINVAL, errbuf, "cm_tr_scan_mx_Create(), do_float and do_int both FALSE.");
+  if (cm->qdbinfo == NULL) return infx_fail(eslEINVAL, errbuf, "cm_tr_scan_mx_Create(), qdbinfo is invalid");
+  CM_TR_SCAN_MX *t = (CM_TR_SCAN_MX *) calloc(1, sizeof(CM_TR_SCAN_MX));
+  t->M = cm->M; t->W = cm->W;
+  t->dnAAA = alloc_bands(cm->M, cm->W);
+  t->dxAAA = alloc_bands(cm->M, cm->W);
+  const CM_QDBINFO *q = cm->qdbinfo;
+  for (int j = 1; j <= t->W; j++)
+    for (int v = 0; v < t->M; v++) {
+      t->dnAAA[SMX_NOQDB][j][v] = t->dnAAA[SMX_QDB1_TIGHT][j][v] = t->dnAAA[SMX_QDB2_LOOSE][j][v] = 1;
+      t->dxAAA[SMX_NOQDB][j][v]      = ESL_MIN(j, t->W);
+      t->dxAAA[SMX_QDB1_TIGHT][j][v] = ESL_MIN(j, ESL_MIN(q->dmax1[v], t->W));
+      t->dxAAA[SMX_QDB2_LOOSE][j][v] = ESL_MIN(j, ESL_MIN(q->dmax2[v], t->W));
+    }
+  t->bestsc   = (float *) malloc(sizeof(float) * (t->W + 1));
+  t->bestr    = (int *) malloc(sizeof(int) * (t->W + 1));
+  t->bestmode = (char *) malloc(t->W + 1);
+  t->floats_valid = do_float ? 1 : 0;
+  t->ints_valid   = do_int ? 1 : 0;
+  *ret_trsmx = t;
+  return eslOK;
+}
+
+extern "C" void cm_tr_scan_mx_Destroy(CM_t *cm, CM_TR_SCAN_MX *t)
+{
+  (void) cm;
+  if (!t) return;
+  infx_dev_release(t->dev);
+  free_bands(t->dnAAA); free_bands(t->dxAAA);
+  free(t->bestsc); free(t->bestr); free(t->bestmode);
+  free(t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* device context                                                            */
+
+static std::mutex g_mu;
+static int  g_state = 0;          /* 0 untried, 1 ok, -1 failed */
+static char g_err[eslERRBUFSIZE];
+static hipStream_t g_stream;
+static hipEvent_t  g_ev0, g_ev1;
+static int  g_ncu = 256;
+static int  g_chunk_override = 0, g_wpb_override = 0;
+
+static thread_local double  t_wall_ms = 0, t_kernel_ms = 0;
+static thread_local int64_t t_cells = 0;
+static thread_local int     t_launches = 0;
+
+extern "C" int infx_dev_init(char *errbuf)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_state == 1) return eslOK;
+  if (g_state == -1) return infx_fail(eslENODEVICE, errbuf, "%s", g_err);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    snprintf(g_err, sizeof(g_err), "no HIP device available (%s); this library has no CPU fallback", hipGetErrorString(e));
+    g_state = -1;
+    return infx_fail(eslENODEVICE, errbuf, "%s", g_err);
+  }
+  hipDeviceProp_t p;
+  if ((e = hipGetDeviceProperties(&p, 0)) != hipSuccess || strncmp(p.gcnArchName, "gfx9", 4) != 0) {
+    snprintf(g_err, sizeof(g_err), "device 0 is not a gfx9 (CDNA) GPU: %s", e == hipSuccess ? p.gcnArchName : hipGetErrorString(e));
+    g_state = -1;
+    return infx_fail(eslENODEVICE, errbuf, "%s", g_err);
+  }
+  g_ncu = p.multiProcessorCount;
+  if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&g_ev0) != hipSuccess || hipEventCreate(&g_ev1) != hipSuccess) {
+    snprintf(g_err, sizeof(g_err), "HIP stream/event creation failed");
+    g_state = -1;
+    return infx_fail(eslENODEVICE, errbuf, "%s", g_err);
+  }
+  g_state = 1;
+  return eslOK;
+}
+
+extern "C" int infx_gpu_available(char *errbuf) { return infx_dev_init(errbuf) == eslOK ? 1 : 0; }
+hipStream_t infx_stream(void) { return g_stream; }
+int infx_num_cu(void) { return g_ncu; }
+
+extern "C" void infx_last_call_timing(double *ret_wall_ms, double *ret_kernel_ms, int64_t *ret_cells, int *ret_launches)
+{
+  if (ret_wall_ms)   *ret_wall_ms = t_wall_ms;
+  if (ret_kernel_ms) *ret_kernel_ms = t_kernel_ms;
+  if (ret_cells)     *ret_cells = t_cells;
+  if (ret_launches)  *ret_launches = t_launches;
+}
+void infx_timing_reset(void) { t_wall_ms = t_kernel_ms = 0; t_cells = 0; t_launches = 0; }
+void infx_timing_add(double wall, double kern, int64_t cells, int launches)
+{
+  t_wall_ms += wall; t_kernel_ms += kern; t_cells += cells; t_launches += launches;
+}
+
+extern "C" void infx_set_launch_params(int chunk_len, int waves_per_block)
+{
+  g_chunk_override = chunk_len > 0 ? chunk_len : 0;
+  g_wpb_override   = (waves_per_block >= 1 && waves_per_block <= 4) ? waves_per_block : 0;
+}
+int infx_chunk_override(void) { return g_chunk_override; }
+int infx_wpb_override(void) { return g_wpb_override; }
+
+/* growable device buffer */
+int DevBuf::reserve(size_t n)
+{
+  if (n <= cap) return eslOK;
+  if (ptr) hipFree(ptr);
+  ptr = NULL; cap = 0;
+  size_t want = n + n / 4 + 256;
+  if (hipMalloc(&ptr, want) != hipSuccess) { ptr = NULL; return eslEMEM; }
+  cap = want;
+  return eslOK;
+}
+void DevBuf::release() { if (ptr) hipFree(ptr); ptr = NULL; cap = 0; }
+
+struct StdProfile {
+  int qdbidx, semi, W;
+  std::vector<DState> st;
+  std::vector<uint32_t> esc, rtmpl, dtmpl;
+  int R, RB;
+  int root_dn, root_dx, root_cnum;
+  int root_child_base[6];
+  uint32_t root_tsc[6];
+  int local_begin;
+  uint32_t el;
+  DevBuf d_st, d_esc, d_rtmpl, d_dtmpl;
+  bool uploaded = false;
+};
+
+struct SmxDev {
+  std::vector<StdProfile *> prof;
+  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i;
+  void  *trdev = NULL;   /* truncated-scanner resources */
+  ~SmxDev() {
+    for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
+    d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release();
+    d_lsum_f.release(); d_lsum_i.release();
+    infx_trdev_release(trdev);
+  }
+};
+
+extern "C" void infx_dev_release(void *dev)
+{
+  if (!dev) return;
+  delete (SmxDev *) dev;
+}
+
+static uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* d = 0 initial values, as cm_scan_mx_InitializeFloats/Integers
+ * (src/cm_mx.c:6039-6138); v = 0..M-1, BEGL_S values for its deck. */
+static void d0_init_float(const CM_t *cm, std::vector<float> &a0)
+{
+  a0.assign(cm->M, (float) IMPOSSIBLE);
+  for (int v = cm->M - 1; v >= 0; v--) {
+    int st = cm->sttype[v];
+    if (cm->stid[v] != BEGL_S) {
+      if (st == E_st) a0[v] = 0.;
+      else if (st == S_st || st == D_st) {
+        int y = cm->cfirst[v];
+        float x = cm->endsc[v];
+        for (int k = 0; k < cm->cnum[v]; k++) x = ESL_MAX(x, (a0[y + k] + cm->tsc[v][k]));
+        x = ESL_MAX(x, IMPOSSIBLE);
+        a0[v] = x;
+      } else if (st == B_st) {
+        a0[v] = a0[cm->cfirst[v]] + a0[cm->cnum[v]];
+      }
+    } else {
+      int y = cm->cfirst[v];
+      float x = cm->endsc[v];
+      for (int k = 0; k < cm->cnum[v]; k++) x = ESL_MAX(x, (a0[y + k] + cm->tsc[v][k]));
+      x = ESL_MAX(x, IMPOSSIBLE);
+      a0[v] = x;
+    }
+  }
+}
+static void d0_init_int(const CM_t *cm, std::vector<int> &a0)
+{
+  a0.assign(cm->M, -INFTY);
+  for (int v = cm->M - 1; v >= 0; v--) {
+    int st = cm->sttype[v];
+    if (cm->stid[v] != BEGL_S) {
+      if (st == E_st) a0[v] = 0;
+      else if (st == S_st || st == D_st) {
+        int y = cm->cfirst[v];
+        int x = cm->iendsc[v];
+        for (int k = 0; k < cm->cnum[v]; k++) x = ESL_MAX(x, (int) ((unsigned) a0[y + k] + (unsigned) cm->itsc[v][k]));
+        x = ESL_MAX(x, -INFTY);
+        a0[v] = x;
+      } else if (st == B_st) {
+        a0[v] = (int) ((unsigned) a0[cm->cfirst[v]] + (unsigned) a0[cm->cnum[v]]);
+      }
+    } else {
+      int y = cm->cfirst[v];
+      int x = cm->iendsc[v];
+      for (int k = 0; k < cm->cnum[v]; k++) x = ESL_MAX(x, (int) ((unsigned) a0[y + k] + (unsigned) cm->itsc[v][k]));
+      x = ESL_MAX(x, -INFTY);
+      a0[v] = x;
+    }
+  }
+}
+
+/* Build the scan profile of <cm> for bands <qdbidx>, semiring <semi> and
+ * effective W (min(smx->W, L)). */
+static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int W, StdProfile &P, char *errbuf)
+{
+  const int M = cm->M;
+  const int *dmin = qdbidx == SMX_QDB1_TIGHT ? cm->qdbinfo->dmin1 : qdbidx == SMX_QDB2_LOOSE ? cm->qdbinfo->dmin2 : NULL;
+  const int *dmax = qdbidx == SMX_QDB1_TIGHT ? cm->qdbinfo->dmax1 : qdbidx == SMX_QDB2_LOOSE ? cm->qdbinfo->dmax2 : NULL;
+  const int banded = (qdbidx != SMX_NOQDB);
+  std::vector<int> dn(M), dx(M), lo(M, INT32_MAX), hi(M, INT32_MIN);
+  for (int v = 0; v < M; v++) { dn[v] = smx->dnAAA[qdbidx][W][v]; dx[v] = smx->dxAAA[qdbidx][W][v]; }
+  auto extend = [&](int v, int a, int b) { if (a > b) return; lo[v] = ESL_MIN(lo[v], a); hi[v] = ESL_MAX(hi[v], b); };
+  auto kbounds = [&](int v, int d, int &kmin, int &kmax) {
+    int w = cm->cfirst[v], y = cm->cnum[v];
+    if (banded) {
+      int dn_y = ESL_MIN(dmin[y], smx->W), dx_y = ESL_MIN(dmax[y], smx->W);
+      int dn_w = ESL_MIN(dmin[w], smx->W), dx_w = ESL_MIN(dmax[w], smx->W);
+      kmin = ESL_MAX(0, ESL_MAX(dn_y, d - dx_w));
+      kmax = ESL_MIN(dx_y, d - dn_w);
+    } else { kmin = 0; kmax = d; }
+  };
+  for (int v = 1; v < M; v++) {
+    int st = cm->sttype[v];
+    if (st == E_st || dn[v] > dx[v]) continue;
+    extend(v, dn[v], dx[v]);
+    if (st == B_st) {
+      int w = cm->cfirst[v], y = cm->cnum[v];
+      for (int d = dn[v]; d <= dx[v]; d++) {
+        int kmin, kmax; kbounds(v, d, kmin, kmax);
+        if (kmin > kmax) continue;
+        extend(y, kmin, kmax);
+        extend(w, d - kmax, d - kmin);
+      }
+    } else {
+      int sd = StateDelta(st);
+      for (int k = 0; k < cm->cnum[v]; k++) extend(cm->cfirst[v] + k, dn[v] - sd, dx[v] - sd);
+    }
+  }
+  if (dn[0] <= dx[0])
+    for (int k = 0; k < cm->cnum[0]; k++) extend(cm->cfirst[0] + k, dn[0], dx[0]);
+
+  std::vector<float> a0f; std::vector<int> a0i;
+  if (semi == SEMI_IINSIDE) d0_init_int(cm, a0i); else d0_init_float(cm, a0f);
+  const uint32_t impbits = (semi == SEMI_IINSIDE) ? (uint32_t) (-INFTY) : fbits((float) IMPOSSIBLE);
+
+  std::vector<int> base(M, 0);
+  P.R = 0; P.RB = 0;
+  P.rtmpl.clear(); P.dtmpl.clear();
+  for (int v = 1; v < M; v++) {
+    if (lo[v] > hi[v]) continue;
+    lo[v] = ESL_MAX(lo[v], 0); hi[v] = ESL_MIN(hi[v], W);
+    if (lo[v] > hi[v]) continue;
+    std::vector<uint32_t> &T = (cm->stid[v] == BEGL_S) ? P.dtmpl : P.rtmpl;
+    int off = (int) T.size();
+    base[v] = off - lo[v];
+    for (int d = lo[v]; d <= hi[v]; d++) {
+      uint32_t b = impbits;
+      if (d == 0) b = (semi == SEMI_IINSIDE) ? (uint32_t) a0i[v] : fbits(a0f[v]);
+      T.push_back(b);
+    }
+  }
+  /* pad to a multiple of 64 words so every slot region stays aligned */
+  while (P.rtmpl.size() % 64) P.rtmpl.push_back(impbits);
+  while (P.dtmpl.size() % 64) P.dtmpl.push_back(impbits);
+  P.R = (int) P.rtmpl.size(); P.RB = (int) P.dtmpl.size();
+
+  P.esc.clear();
+  P.st.assign(M, DState());
+  for (int v = 0; v < M; v++) {
+    DState &s = P.st[v];
+    memset(&s, 0, sizeof(DState));
+    int st = cm->sttype[v];
+    s.kind = (st == E_st) ? KIND_SKIP : (st == B_st) ? KIND_B : (cm->stid[v] == BEGL_S) ? KIND_BEGL : (st == IL_st) ? KIND_IL : KIND_GENERIC;
+    if (v > 0 && s.kind != KIND_SKIP && cm->sttype[v] == IL_st && cm->cfirst[v] != v)
+      return infx_fail(eslEINVAL, errbuf, "IL state %d does not loop on itself", v);
+    s.emitmode = Emitmode(st);
+    s.sd = StateDelta(st);
+    s.rdelta = StateRightDelta(st) > 0;
+    s.cnum = (st == B_st) ? 0 : cm->cnum[v];
+    s.dn = dn[v]; s.dx = dx[v];
+    s.base = base[v];
+    s.esc_off = 0;
+    if (st == ML_st || st == MR_st || st == IL_st || st == IR_st || st == MP_st) {
+      int n = (st == MP_st) ? INFX_KP * INFX_KP : INFX_KP;
+      s.esc_off = (int) P.esc.size();
+      for (int x = 0; x < n; x++) P.esc.push_back(semi == SEMI_IINSIDE ? (uint32_t) cm->ioesc[v][x] : fbits(cm->oesc[v][x]));
+    }
+    if (semi == SEMI_IINSIDE) {
+      s.has_begin = cm->ibeginsc[v] != -INFTY;
+      s.beginsc = (uint32_t) cm->ibeginsc[v];
+      s.init_ok = cm->iendsc[v] != -INFTY;
+      s.endsc = (uint32_t) cm->iendsc[v];
+      if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = (uint32_t) cm->itsc[v][k];
+    } else {
+      s.has_begin = NOT_IMPOSSIBLE(cm->beginsc[v]);
+      s.beginsc = fbits(cm->beginsc[v]);
+      s.init_ok = NOT_IMPOSSIBLE(cm->endsc[v]);
+      s.endsc = fbits(cm->endsc[v]);
+      if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = fbits(cm->tsc[v][k]);
+    }
+    if (st == B_st) {
+      int w = cm->cfirst[v], y = cm->cnum[v];
+      s.child_base[0] = base[w];
+      s.child_base[1] = base[y];
+      s.banded = banded;
+      if (banded) {
+        s.kdn_y = ESL_MIN(dmin[y], smx->W); s.kdx_y = ESL_MIN(dmax[y], smx->W);
+        s.kdn_w = ESL_MIN(dmin[w], smx->W); s.kdx_w = ESL_MIN(dmax[w], smx->W);
+      }
+    } else if (st != E_st) {
+      for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.child_base[k] = base[cm->cfirst[v] + k];
+    }
+  }
+  P.root_dn = dn[0]; P.root_dx = dx[0];
+  P.root_cnum = cm->cnum[0];
+  if (P.root_cnum < 1 || P.root_cnum > 6) return infx_fail(eslEINVAL, errbuf, "ROOT_S has %d children", P.root_cnum);
+  for (int k = 0; k < 6; k++) {
+    P.root_child_base[k] = (k < P.root_cnum) ? base[cm->cfirst[0] + k] : 0;
+    P.root_tsc[k] = (k < P.root_cnum) ? (semi == SEMI_IINSIDE ? (uint32_t) cm->itsc[0][k] : fbits(cm->tsc[0][k])) : 0;
+  }
+  P.local_begin = (cm->flags & CMH_LOCAL_BEGIN) ? 1 : 0;
+  P.el = (semi == SEMI_IINSIDE) ? (uint32_t) cm->iel_selfsc : fbits(cm->el_selfsc);
+  if (P.esc.empty()) P.esc.push_back(0);
+  P.qdbidx = qdbidx; P.semi = semi; P.W = W;
+  return eslOK;
+}
+
+static int upload(DevBuf &b, const void *src, size_t n)
+{
+  int st;
+  if ((st = b.reserve(n)) != eslOK) return st;
+  return hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, g_stream) == hipSuccess ? eslOK : eslESYS;
+}
+
+static bool same_profile(const StdProfile &a, const StdProfile &b)
+{
+  return a.st.size() == b.st.size() && memcmp(a.st.data(), b.st.data(), a.st.size() * sizeof(DState)) == 0 &&
+         a.esc == b.esc && a.rtmpl == b.rtmpl && a.dtmpl == b.dtmpl &&
+         a.root_dn == b.root_dn && a.root_dx == b.root_dx && a.root_cnum == b.root_cnum &&
+         memcmp(a.root_child_base, b.root_child_base, sizeof(a.root_child_base)) == 0 &&
+         memcmp(a.root_tsc, b.root_tsc, sizeof(a.root_tsc)) == 0 && a.local_begin == b.local_begin && a.el == b.el;
+}
+
+/* ------------------------------------------------------------------------ */
+/* the standard scanners                                                      */
+
+static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,
+                    int64_t i0, int64_t j0, float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                    int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+{
+  int status;
+  auto t_start = std::chrono::steady_clock::now();
+  infx_timing_reset();
+  /* contract checks (src/cm_dpsearch.c:129-135, :1146-1151, :1771-1776) */
+  if (!(cm->flags & CMH_BITS))        return infx_fail(eslEINCOMPAT, errbuf, "%s, CMH_BITS flag is not raised.\n", fname);
+  if (j0 < i0)                        return infx_fail(eslEINCOMPAT, errbuf, "%s, i0: %lld j0: %lldd\n", fname, (long long) i0, (long long) j0);
+  if (dsq == NULL)                    return infx_fail(eslEINCOMPAT, errbuf, "%s, dsq is NULL\n", fname);
+  if (semi == SEMI_CYK && (cm->search_opts & CM_SEARCH_INSIDE))     return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag raised", fname);
+  if (semi != SEMI_CYK && !(cm->search_opts & CM_SEARCH_INSIDE))    return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag not raised", fname);
+  if (smx == NULL)                    return infx_fail(eslEINCOMPAT, errbuf, "%s, smx == NULL\n", fname);
+  if (semi == SEMI_CYK && !smx->floats_valid) return infx_fail(eslEINCOMPAT, errbuf, "%s, smx->floats_valid if FALSE", fname);
+  if (semi != SEMI_CYK && !smx->ints_valid)   return infx_fail(eslEINCOMPAT, errbuf, "%s, smx->ints_valid if FALSE", fname);
+  if (cm->qdbinfo == NULL)            return infx_fail(eslEINCOMPAT, errbuf, "%s, cm->qdbinfo == NULL\n", fname);
+  if (qdbidx < 0 || qdbidx > 2)       return infx_fail(eslEINCOMPAT, errbuf, "%s, qdbidx is invalid", fname);
+  if (i0 < 1)                         return infx_fail(eslEINCOMPAT, errbuf, "%s, i0 must be >= 1", fname);
+  if (j0 - i0 + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
+  for (int64_t x = i0; x <= j0; x++)
+    if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
+  if ((status = infx_dev_init(errbuf)) != eslOK) return status;
+
+  const int64_t L = j0 - i0 + 1;
+  const int W = (smx->W > L) ? (int) L : smx->W;
+  if (!smx->dev) smx->dev = new SmxDev();
+  SmxDev *D = (SmxDev *) smx->dev;
+
+  /* profile */
+  StdProfile fresh;
+  if ((status = build_std_profile(cm, smx, qdbidx, semi, W, fresh, errbuf)) != eslOK) return status;
+  StdProfile *P = NULL;
+  for (auto *p : D->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->W == W) P = p;
+  if (!P) { P = new StdProfile(); D->prof.push_back(P); }
+  if (!P->uploaded || !same_profile(*P, fresh)) {
+    P->st = fresh.st; P->esc = fresh.esc; P->rtmpl = fresh.rtmpl; P->dtmpl = fresh.dtmpl;
+    P->R = fresh.R; P->RB = fresh.RB; P->root_dn = fresh.root_dn; P->root_dx = fresh.root_dx; P->root_cnum = fresh.root_cnum;
+    memcpy(P->root_child_base, fresh.root_child_base, sizeof(P->root_child_base));
+    memcpy(P->root_tsc, fresh.root_tsc, sizeof(P->root_tsc));
+    P->local_begin = fresh.local_begin; P->el = fresh.el;
+    P->qdbidx = qdbidx; P->semi = semi; P->W = W;
+    if ((status = upload(P->d_st, P->st.data(), P->st.size() * sizeof(DState))) != eslOK ||
+        (status = upload(P->d_esc, P->esc.data(), P->esc.size() * 4)) != eslOK ||
+        (status = upload(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4 + 4)) != eslOK ||
+        (status = upload(P->d_dtmpl, P->dtmpl.data(), P->dtmpl.size() * 4 + 4)) != eslOK)
+      return infx_fail(status, errbuf, "%s: device upload failed", fname);
+    P->uploaded = true;
+  }
+  if (semi == SEMI_FINSIDE && D->d_lsum_f.cap == 0) {
+    if ((status = upload(D->d_lsum_f, infx_flogsum_table(), LOGSUM_TBL * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+  }
+  if (semi == SEMI_IINSIDE && D->d_lsum_i.cap == 0) {
+    if ((status = upload(D->d_lsum_i, infx_ilogsum_table(), LOGSUM_TBL * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+  }
+
+  /* work split: chunks of the target, >= 2W long so the W-1 warm-up stays small */
+  const int wpb = infx_wpb_override() ? infx_wpb_override() : 4;
+  int64_t chunk = infx_chunk_override();
+  if (chunk <= 0) {
+    int64_t target = (int64_t) g_ncu * 8;
+    chunk = (L + target - 1) / target;
+    chunk = ESL_MAX(chunk, (int64_t) 2 * W);
+    chunk = ESL_MAX(chunk, (int64_t) 64);
+  }
+  const int64_t nchunks64 = (L + chunk - 1) / chunk;
+  const int nchunks = (int) nchunks64;
+  const size_t slot_bytes = (((size_t) 2 * P->R + (size_t) (W + 1) * P->RB + 2 * (size_t) (W + 1) + (size_t) cm->M) * 4 + 255) / 256 * 256;
+  const size_t scratch_budget = (size_t) 24 << 30;
+  int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
+  if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+
+  if ((status = D->d_dsq.reserve((size_t) L + 64)) != eslOK ||
+      (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
+      (status = D->d_chunk.reserve((size_t) nchunks * 24 + 64)) != eslOK ||
+      (status = D->d_ncand.reserve(64)) != eslOK)
+    return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
+  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
+  if ((status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
+  cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
+  if (hipMemcpyAsync(D->d_dsq.ptr, dsq + i0, (size_t) L, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
+
+  ScanArgs A;
+  memset(&A, 0, sizeof(A));
+  A.st = (const DState *) P->d_st.ptr;
+  A.esc = P->d_esc.ptr;
+  A.dsq = (const uint8_t *) D->d_dsq.ptr;
+  A.dsq_off = i0;
+  A.row_tmpl = P->d_rtmpl.ptr;
+  A.deck_tmpl = P->d_dtmpl.ptr;
+  A.lsum = (semi == SEMI_FINSIDE) ? D->d_lsum_f.ptr : (semi == SEMI_IINSIDE) ? D->d_lsum_i.ptr : NULL;
+  A.scratch = (char *) D->d_scratch.ptr;
+  A.slot_bytes = (int64_t) slot_bytes;
+  A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB;
+  A.i0 = i0; A.j0 = j0;
+  A.chunk_len = chunk; A.nchunks = nchunks; A.nslots = nslots;
+  A.root_dn = P->root_dn; A.root_dx = P->root_dx; A.root_cnum = P->root_cnum;
+  memcpy(A.root_child_base, P->root_child_base, sizeof(A.root_child_base));
+  memcpy(A.root_tsc, P->root_tsc, sizeof(A.root_tsc));
+  A.local_begin = P->local_begin;
+  A.el_selfsc = P->el;
+  const int greedy = !(cm->search_opts & CM_SEARCH_CMNOTGREEDY);
+  A.cand_mode = (hitlist == NULL) ? 0 : (!greedy ? 2 : (do_null3 ? 0 : 1));
+  A.cutoff = (hitlist == NULL) ? INFINITY : cutoff;
+  A.env_cutoff = env_cutoff;
+  A.cand = (Cand *) D->d_cand.ptr;
+  A.ncand = (unsigned long long *) D->d_ncand.ptr;
+  A.cand_cap = cand_cap;
+  A.chunk_best = (float *) D->d_chunk.ptr;
+  A.chunk_envi = (int64_t *) ((char *) D->d_chunk.ptr + (((size_t) nchunks * 4 + 255) / 256) * 256);
+  A.chunk_envj = A.chunk_envi + nchunks;
+  A.want_vsc = (ret_vsc != NULL);
+  if ((status = D->d_chunk.reserve(((((size_t) nchunks * 4 + 255) / 256) * 256) + (size_t) nchunks * 16 + 64)) != eslOK)
+    return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
+  A.chunk_best = (float *) D->d_chunk.ptr;
+  A.chunk_envi = (int64_t *) ((char *) D->d_chunk.ptr + (((size_t) nchunks * 4 + 255) / 256) * 256);
+  A.chunk_envj = A.chunk_envi + nchunks;
+
+  dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
+  unsigned long long ncand = 0;
+  float kernel_ms_total = 0;
+  int launches = 0;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    if (hipMemsetAsync(A.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
+    hipEventRecord(g_ev0, g_stream);
+    hipError_t e = infx_launch_std(semi, A.want_vsc, grid, block, g_stream, &A);
+    hipEventRecord(g_ev1, g_stream);
+    if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
+    if (hipMemcpyAsync(&ncand, A.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess)
+      return infx_fail(eslESYS, errbuf, "%s: kernel execution failed: %s", fname, hipGetErrorString(hipGetLastError()));
+    float ms = 0; hipEventElapsedTime(&ms, g_ev0, g_ev1);
+    kernel_ms_total += ms; launches++;
+    if ((int64_t) ncand <= cand_cap) break;
+    if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer allocation failed", fname);
+    cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
+    A.cand = (Cand *) D->d_cand.ptr; A.cand_cap = cand_cap;
+  }
+  if ((int64_t) ncand > cand_cap) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer overflow", fname);
+
+  std::vector<Cand> hc(ncand);
+  std::vector<float> cbest(nchunks);
+  std::vector<int64_t> cenvi(nchunks), cenvj(nchunks);
+  if ((ncand && hipMemcpyAsync(hc.data(), A.cand, ncand * sizeof(Cand), hipMemcpyDeviceToHost, g_stream) != hipSuccess) ||
+      hipMemcpyAsync(cbest.data(), A.chunk_best, nchunks * 4, hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+      hipMemcpyAsync(cenvi.data(), A.chunk_envi, nchunks * 8, hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+      hipMemcpyAsync(cenvj.data(), A.chunk_envj, nchunks * 8, hipMemcpyDeviceToHost, g_stream) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "%s: result download failed", fname);
+  std::vector<float> vsc_slots;
+  if (ret_vsc) {
+    vsc_slots.resize((size_t) nslots * cm->M);
+    size_t vsc_off = ((size_t) 2 * P->R + (size_t) (W + 1) * P->RB + 2 * (size_t) (W + 1)) * 4;
+    if (hipMemcpy2DAsync(vsc_slots.data(), (size_t) cm->M * 4, A.scratch + vsc_off, slot_bytes, (size_t) cm->M * 4, nslots,
+                         hipMemcpyDeviceToHost, g_stream) != hipSuccess)
+      return infx_fail(eslESYS, errbuf, "%s: vsc download failed", fname);
+  }
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: download failed", fname);
+
+  float vsc_root = IMPOSSIBLE;
+  int64_t envi = INT64_MAX, envj = INT64_MIN;
+  for (int c = 0; c < nchunks; c++) {
+    vsc_root = ESL_MAX(vsc_root, cbest[c]);
+    envi = ESL_MIN(envi, cenvi[c]);
+    envj = ESL_MAX(envj, cenvj[c]);
+  }
+  if (ret_envi) *ret_envi = (envi == INT64_MAX) ? -1 : envi;
+  if (ret_envj) *ret_envj = (envj == INT64_MIN) ? -1 : envj;
+  if (ret_sc) *ret_sc = vsc_root;
+  if (ret_vsc) {
+    float *vsc = (float *) malloc(sizeof(float) * cm->M);
+    for (int v = 0; v < cm->M; v++) {
+      float m = IMPOSSIBLE;
+      for (int s = 0; s < nslots; s++) m = ESL_MAX(m, vsc_slots[(size_t) s * cm->M + v]);
+      vsc[v] = m;
+    }
+    vsc[0] = vsc_root;
+    *ret_vsc = vsc;
+  }
+
+  if (hitlist) {
+    std::vector<INFX_CAND> cands(ncand);
+    for (size_t k = 0; k < ncand; k++) { cands[k].j = hc[k].j; cands[k].d = hc[k].d; cands[k].sc = hc[k].sc; cands[k].r = hc[k].r; cands[k].mode = TRMODE_J; }
+    if ((status = infx_resolve_hits(cm, errbuf, PLI_PASS_STD_ANY, 0, cands.data(), (int64_t) ncand, dsq, i0, j0, W, cutoff, do_null3,
+                                    greedy, hitlist)) != eslOK)
+      return status;
+  }
+
+  double cells = 0;
+  infx_count_scan_cells(cm, smx, qdbidx, 0, i0, j0, &cells);
+  double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  infx_timing_add(wall, kernel_ms_total, (int64_t) cells, launches);
+  return eslOK;
+}
+
+/* Exact number of (v, j, d) cells the reference computes over j = i0..j0
+ * (E states excluded, v = 0 included): bands clipped by jp for jp < W. */
+extern "C" int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells)
+{
+  int M, Wm; int ***dnA, ***dxA;
+  if (truncated) { const CM_TR_SCAN_MX *t = (const CM_TR_SCAN_MX *) mx; M = t->M; Wm = t->W; dnA = t->dnAAA; dxA = t->dxAAA; }
+  else           { const CM_SCAN_MX *s = (const CM_SCAN_MX *) mx;    M = s->M; Wm = s->W; dnA = s->dnAAA; dxA = s->dxAAA; }
+  const int64_t L = j0 - i0 + 1;
+  const int W = (Wm > L) ? (int) L : Wm;
+  double cells = 0;
+  /* rows 1..W-1 once each, row W for every j with jp >= W; E states are skipped by the scanners */
+  for (int jp = 1; jp <= W; jp++) {
+    double mult = (jp < W) ? 1.0 : (double) (L - W + 1);
+    double row = 0;
+    for (int v = 0; v < M; v++) {
+      if (cm->sttype[v] == E_st) continue;
+      int dn = dnA[qdbidx][jp][v], dx = dxA[qdbidx][jp][v];
+      if (dx >= dn) row += dx - dn + 1;
+    }
+    cells += mult * row;
+  }
+  *ret_cells = cells;
+  return eslOK;
+}
+
+extern "C" int FastCYKScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                           float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                           int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+{
+  return std_scan(SEMI_CYK, "FastCYKScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+                  ret_envi, ret_envj, ret_vsc, ret_sc);
+}
+extern "C" int FastFInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                               float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                               int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+{
+  return std_scan(SEMI_FINSIDE, "FastFInsideScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+                  ret_envi, ret_envj, ret_vsc, ret_sc);
+}
+extern "C" int FastIInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                               float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                               int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+{
+  return std_scan(SEMI_IINSIDE, "FastIInsideScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+                  ret_envi, ret_envj, ret_vsc, ret_sc);
+}

@@ -1,0 +1,29 @@
+/* scan_host.h -- host-side launch helpers shared by the standard and the
+ * truncated GPU scanners. */
+#ifndef INFX_SCAN_HOST_H
+#define INFX_SCAN_HOST_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+struct DevBuf {
+  void  *ptr = nullptr;
+  size_t cap = 0;
+  int  reserve(size_t n);   /* grow (contents not kept) */
+  void release();
+};
+
+extern "C" int  infx_dev_init(char *errbuf);
+extern "C" void infx_dev_release(void *dev);
+extern "C" void infx_trdev_release(void *trdev);
+extern "C" int  infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells);
+
+hipStream_t infx_stream(void);
+int  infx_num_cu(void);
+int  infx_chunk_override(void);
+int  infx_wpb_override(void);
+void infx_timing_reset(void);
+void infx_timing_add(double wall, double kern, int64_t cells, int launches);
+
+#endif

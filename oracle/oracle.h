@@ -1,0 +1,60 @@
+/* oracle.h -- CPU restatement of Infernal's QDB scanners.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker the GPU scanners are
+ * compared against (tests/, __graft_entry__.smoke(), and bench.py's CPU
+ * baseline).  The product library never links, loads or calls it.
+ *
+ * It restates, on the reference's own data layout (full alpha[2][M][W+1]
+ * rows, alpha_begl[W+1][nbegl][W+1] decks) and in the reference's operation
+ * order, the scanners of src/cm_dpsearch.c and src/cm_dpsearch_trunc.c, the
+ * scan-matrix set-up of src/cm_mx.c, the hit reporters of src/cm_mx.c and the
+ * overlap removal of src/cm_tophits.c.  It reads a configured model through
+ * the CM_t mirror declared in include/infernal_gpu.h.
+ */
+#ifndef ORACLE_H
+#define ORACLE_H
+
+#include <stdint.h>
+#include "../include/infernal_gpu.h"
+
+#define ORACLE_CYK      0
+#define ORACLE_FINSIDE  1
+#define ORACLE_IINSIDE  2
+#define ORACLE_TRCYK    3
+#define ORACLE_TRIINSIDE 4
+
+typedef struct oracle_hit_s {
+  int64_t start, stop;
+  int     root;
+  int     mode;
+  float   score;
+} ORACLE_HIT;
+
+typedef struct oracle_result_s {
+  int         nhits;
+  ORACLE_HIT *hits;          /* kept hits, in the order they are appended to the caller's list */
+  int64_t     envi, envj;    /* -1 if no cell reached env_cutoff */
+  float       sc;            /* best score over all (j, d) */
+  int         mode;          /* truncated scanners: mode of the best hit */
+  float      *vsc;           /* [M] best score per state */
+  /* optional per-j dump of the root row: for each j in i0..j0 and each
+   * d = 0..W, bestsc and bestr; allocated only if want_rows was set */
+  int         W;
+  float      *rows_sc;       /* [(j-i0)*(W+1) + d] */
+  int        *rows_r;
+  char       *rows_mode;
+} ORACLE_RESULT;
+
+/* Run one scan.  algo: ORACLE_*; greedy: 1 greedy overlap removal, 0 the
+ * gamma semi-HMM; pass_idx is used by the truncated scanners.  Returns eslOK
+ * or an Easel-style error code (message in errbuf). */
+int  oracle_scan(const CM_t *cm, int algo, int qdbidx, int pass_idx, const ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                 float cutoff, int do_null3, int greedy, float env_cutoff, int want_rows,
+                 ORACLE_RESULT *res, char *errbuf);
+void oracle_result_free(ORACLE_RESULT *res);
+
+/* The oracle's own log-sum tables (src/logsum.c:74-148), for table checks. */
+const float *oracle_flogsum_table(void);
+const int   *oracle_ilogsum_table(void);
+
+#endif

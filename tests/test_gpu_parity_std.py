@@ -1,0 +1,108 @@
+"""GPU standard scanners vs the oracle, through the C ABI.
+
+Bar: FastCYKScan and FastIInsideScan bit-exact (scores, hit lists, envelopes,
+per-state maxima); FastFInsideScan within 0.01 bits (it is bit-exact in
+practice, because the GPU follows the same FLogsum order and table).
+"""
+import numpy as np
+import pytest
+
+import infernal_amd as ia
+import oracle_bind as ob
+from helpers import load_cm, random_dsq, hits_equal, describe_hit_diff
+
+pytestmark = pytest.mark.gpu
+
+FINSIDE_TOL = 0.01
+
+
+def _run(fn_gpu, algo, cm, qdb, dsq, i0, j0, cutoff, null3=False, greedy=True, env=0.0, vsc=False, tol=0.0):
+    print("case", fn_gpu.__name__, cm.c.name.decode(), qdb, i0, j0, cutoff, null3, greedy, flush=True)
+    g = fn_gpu(cm, qdb, dsq, i0, j0, cutoff, do_null3=null3, env_cutoff=env, want_vsc=vsc, greedy=greedy)
+    o = ob.scan(cm, algo, qdb, dsq, i0, j0, cutoff, do_null3=null3, greedy=greedy, env_cutoff=env)
+    if tol == 0.0:
+        assert np.float32(g["sc"]).view(np.uint32) == np.float32(o["sc"]).view(np.uint32), (g["sc"], o["sc"])
+        assert hits_equal(g["hits"], o["hits"]), describe_hit_diff(g["hits"], o["hits"])
+        if vsc:
+            assert np.array_equal(g["vsc"].view(np.uint32), o["vsc"].view(np.uint32))
+    else:
+        assert abs(g["sc"] - o["sc"]) <= tol
+        assert len(g["hits"]) == len(o["hits"])
+        for x, y in zip(g["hits"], o["hits"]):
+            assert x[:4] == y[:4] and abs(x[4] - y[4]) <= tol
+    assert (g["envi"], g["envj"]) == (o["envi"], o["envj"])
+    return g, o
+
+
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Vault.c.cm", "5S_rRNA.c.cm"])
+@pytest.mark.parametrize("qdb", [1, 2])
+def test_cyk_local(name, qdb):
+    cm = load_cm(name)
+    dsq = random_dsq(3000, seed=11 + qdb)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, qdb, dsq, 1, 3000, 0.0, env=5.0, vsc=True)
+
+
+def test_cyk_glocal_and_noqdb():
+    cm = load_cm("tRNA.c.cm", local=False)
+    dsq = random_dsq(1200, seed=5)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 0, dsq, 1, 1200, -20.0, vsc=True)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 1200, -20.0)
+
+
+def test_cyk_chunked_long():
+    """Many chunks: the W-1 warm-up must reproduce the reference cells."""
+    cm = load_cm("tRNA.c.cm")
+    dsq = random_dsq(20000, seed=3)
+    ia.set_launch_params(chunk_len=500)
+    try:
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 20000, 0.0, env=3.0)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 777, 19000, 0.0, env=3.0)
+    finally:
+        ia.set_launch_params(0, 0)
+
+
+def test_cyk_null3_and_gamma():
+    cm = load_cm("Vault.c.cm")
+    dsq = random_dsq(4000, seed=9, gc=0.6)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 4000, 0.0, null3=True)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 4000, 0.0, greedy=False)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 4000, 0.0, null3=True, greedy=False)
+
+
+def test_short_targets():
+    cm = load_cm("tRNA.c.cm")
+    for L in (1, 2, 7, 50, 217, 218, 219):
+        dsq = random_dsq(L, seed=100 + L)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, L, -50.0, vsc=True)
+        _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 1, dsq, 1, L, -50.0, vsc=True)
+
+
+def test_degenerate_residues():
+    cm = load_cm("5S_rRNA.c.cm")
+    dsq = random_dsq(2500, seed=21, degenerate=0.05)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 2500, 0.0)
+    _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 2, dsq, 1, 2500, 0.0)
+
+
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Vault.c.cm"])
+def test_finside(name):
+    cm = load_cm(name)
+    dsq = random_dsq(3000, seed=31)
+    _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 3000, 0.0, env=5.0, vsc=True)
+    _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 3000, 0.0, env=5.0, tol=FINSIDE_TOL)
+
+
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Vault.c.cm"])
+def test_iinside(name):
+    cm = load_cm(name)
+    dsq = random_dsq(3000, seed=41)
+    _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 2, dsq, 1, 3000, 0.0, env=5.0, vsc=True)
+    _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 1, dsq, 1, 3000, 0.0, null3=True)
+
+
+def test_bifurcating_models():
+    for name in ("Cobalamin.c.cm", "rnaseP-eubact.c.cm"):
+        cm = load_cm(name)
+        dsq = random_dsq(2500, seed=51)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 2500, 0.0, vsc=True)
+        _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 1500, 0.0)
