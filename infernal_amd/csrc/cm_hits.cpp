@@ -170,13 +170,10 @@ extern "C" int cm_TrFillFromPassIdx(int pass_idx, int *ret_fill_L, int *ret_fill
 extern "C" int cm_tr_penalties_IdxForPass(int pass_idx)
 {
   switch (pass_idx) {
-  case PLI_PASS_CM_SUMMED:       return TRPENALTY_5P_AND_3P;
-  case PLI_PASS_STD_ANY:         return TRPENALTY_5P_AND_3P;
   case PLI_PASS_5P_ONLY_FORCE:   return TRPENALTY_5P_ONLY;
   case PLI_PASS_3P_ONLY_FORCE:   return TRPENALTY_3P_ONLY;
   case PLI_PASS_5P_AND_3P_FORCE: return TRPENALTY_5P_AND_3P;
   case PLI_PASS_5P_AND_3P_ANY:   return TRPENALTY_5P_AND_3P;
-  case PLI_PASS_HMM_ONLY_ANY:    return TRPENALTY_5P_AND_3P;
   }
   return -1;
 }

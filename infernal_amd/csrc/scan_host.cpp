@@ -409,11 +409,16 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
   return eslOK;
 }
 
-static int upload(DevBuf &b, const void *src, size_t n)
+static hipError_t g_last_hip = hipSuccess;
+
+/* copy n bytes to a device buffer of at least n+pad bytes */
+static int upload(DevBuf &b, const void *src, size_t n, size_t pad = 0)
 {
   int st;
-  if ((st = b.reserve(n)) != eslOK) return st;
-  return hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, g_stream) == hipSuccess ? eslOK : eslESYS;
+  if ((st = b.reserve(n + pad)) != eslOK) return st;
+  if (n == 0) return eslOK;
+  g_last_hip = hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, g_stream);
+  return g_last_hip == hipSuccess ? eslOK : eslESYS;
 }
 
 static bool same_profile(const StdProfile &a, const StdProfile &b)
@@ -472,9 +477,9 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     P->qdbidx = qdbidx; P->semi = semi; P->W = W;
     if ((status = upload(P->d_st, P->st.data(), P->st.size() * sizeof(DState))) != eslOK ||
         (status = upload(P->d_esc, P->esc.data(), P->esc.size() * 4)) != eslOK ||
-        (status = upload(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4 + 4)) != eslOK ||
-        (status = upload(P->d_dtmpl, P->dtmpl.data(), P->dtmpl.size() * 4 + 4)) != eslOK)
-      return infx_fail(status, errbuf, "%s: device upload failed", fname);
+        (status = upload(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4, 4)) != eslOK ||
+        (status = upload(P->d_dtmpl, P->dtmpl.data(), P->dtmpl.size() * 4, 4)) != eslOK)
+      return infx_fail(status, errbuf, "%s: device upload failed: %s", fname, hipGetErrorString(g_last_hip));
     P->uploaded = true;
   }
   if (semi == SEMI_FINSIDE && D->d_lsum_f.cap == 0) {

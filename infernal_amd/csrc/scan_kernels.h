@@ -88,4 +88,82 @@ struct ScanArgs {
   int            want_vsc;
 };
 
+/* ---------------------------------------------------------------------- */
+/* truncated scanners                                                      */
+
+enum { TSEMI_CYK = 0, TSEMI_IINSIDE = 1 };
+#define TRMODE_T_K 0
+#define TRMODE_R_K 1
+#define TRMODE_L_K 2
+#define TRMODE_J_K 3
+#define TRMODE_UNKNOWN_K 4
+
+/* 128 bytes */
+struct TState {
+  int32_t kind;
+  int32_t emitmode;
+  int32_t cnum;
+  int32_t dx;          /* band max (dn = 1) */
+  int32_t base;        /* J/L/R rows (or deck rows for BEGL): cell d at base + d */
+  int32_t tbase;       /* T rows (B states) */
+  int32_t esc_off;     /* oesc/ioesc row */
+  int32_t lesc_off;    /* lmesc row (MP) */
+  int32_t resc_off;    /* rmesc row (MP) */
+  int32_t detached;
+  int32_t first_r;     /* first child used in the L/R marginal (1 for IL/IR: no self transition) */
+  int32_t init_ok;
+  int32_t banded;
+  int32_t kdx_y, kdx_w;
+  uint32_t endsc;
+  uint32_t tsc[6];
+  int32_t child_base[6];
+  int32_t pad[4];
+};
+
+/* a state with a finite truncated-begin penalty, in ascending order */
+struct TrPen {
+  int32_t y, base, tbase, dx, is_b;
+  uint32_t pen;
+};
+
+struct TrWindow {
+  int64_t i0, j0;
+  int64_t dsq_off;     /* device dsq + dsq_off, indexed by absolute j */
+};
+
+struct TrCand {
+  int32_t win, j, d;
+  float   sc;
+  int32_t r, mode;
+};
+
+struct TrScanArgs {
+  const TState  *st;
+  const void    *esc;
+  const uint8_t *dsq;
+  const void    *row_tmpl;   /* [3][R] J, L, R */
+  const void    *deck_tmpl;  /* [3][RB] */
+  const void    *lsum;
+  const TrPen   *pen;
+  int            npen;
+  const TrWindow *win;
+  int            nwin;
+  char          *scratch;
+  int64_t        slot_bytes;
+  int            nslots;
+  int            M, W, R, RB, RT;
+  int            root_dx;
+  int            fill_L, fill_R, fill_T;
+  uint32_t       el_selfsc;
+  int            cand_mode;   /* 0: sc >= cutoff, 2: sc > 0 (gamma) */
+  float          cutoff, env_cutoff;
+  TrCand        *cand;
+  unsigned long long *ncand;
+  int64_t        cand_cap;
+  float         *win_best;
+  int           *win_mode;
+  int64_t       *win_envi, *win_envj;
+  float         *win_vsc;     /* [nwin][M] */
+};
+
 #endif

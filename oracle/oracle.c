@@ -466,4 +466,34 @@ void oracle_result_free(ORACLE_RESULT *res)
   memset(res, 0, sizeof(*res));
 }
 
-/* truncated scanners: oracle_trunc.c */
+/* ------------------------------------------------------------------------ */
+/* helpers for the truncated scanners (oracle_trunc.c)                        */
+
+int oracle_trunc_hits_begin(void **acc, int L, int64_t i0, int64_t j0, int W, int greedy)
+{
+  HITACC *a = (HITACC *) malloc(sizeof(HITACC));
+  hitacc_init(a, L, i0, j0, W, greedy);
+  *acc = a;
+  return eslOK;
+}
+int oracle_trunc_hits_update(void *acc, const CM_t *cm, int pass_idx, int j, int dmin, int dmax, const float *bestsc,
+                             const int *bestr, const char *bestmode, const double *act, float cutoff, int do_null3)
+{
+  return hitacc_update((HITACC *) acc, cm, pass_idx, j, dmin, dmax, bestsc, bestr, bestmode, act, cutoff, do_null3);
+}
+int oracle_trunc_hits_finish(void *acc, ORACLE_RESULT *res, char *errbuf)
+{
+  HITACC *a = (HITACC *) acc;
+  int st = res ? hitacc_finish(a, res, errbuf) : eslOK;
+  hitacc_free(a);
+  free(a);
+  return st;
+}
+/* act[jp % (W+1)] = act[(jp-1) % (W+1)] + count(x) (src/cm_dpsearch.c:223-230) */
+void oracle_act_update(double *act, int W, int jp, ESL_DSQ x)
+{
+  double *dst = &act[(size_t) (jp % (W + 1)) * INFX_K];
+  const double *src = &act[(size_t) ((jp - 1) % (W + 1)) * INFX_K];
+  for (int a = 0; a < INFX_K; a++) dst[a] = src[a];
+  oracle_DCount(dst, x, 1.);
+}

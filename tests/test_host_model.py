@@ -1,0 +1,197 @@
+"""Host-side model input and configuration, pinned against the .cm files'
+own printed scores and against closed forms (no GPU needed)."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import infernal_amd as ia
+from helpers import CMDIR, ROOT
+
+ALL_CMS = sorted(f for f in os.listdir(CMDIR) if f.endswith(".cm"))
+
+
+def _file_state_lines(path):
+    """(type, fields) of every state line, in order."""
+    out = []
+    in_cm = False
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if not t:
+                continue
+            if t[0] == "CM":
+                in_cm = True
+                continue
+            if not in_cm or t[0] == "[":
+                continue
+            if t[0] == "//":
+                break
+            out.append(t)
+    return out
+
+
+@pytest.mark.parametrize("name", ALL_CMS)
+def test_read_matches_file(name):
+    """Global-mode log-odds scores reproduce the file's %.3f values
+    (src/cm_file.c:ascii2prob, src/cm.c:CMRenormalize and CMLogoddsify)."""
+    path = os.path.join(CMDIR, name)
+    cm = ia.CM.read(path)
+    cm.configure(local=False)
+    c = cm.c
+    lines = _file_state_lines(path)
+    assert len(lines) == c.M
+    nullv = [c.null[x] for x in range(4)]
+    for v, t in enumerate(lines):
+        st = ord(c.sttype[v])
+        assert int(t[1]) == v
+        assert c.cfirst[v] == int(t[4]) and c.cnum[v] == int(t[5])
+        q = c.qdbinfo.contents
+        assert (q.dmin2[v], q.dmin1[v], q.dmax1[v], q.dmax2[v]) == tuple(int(x) for x in t[6:10])
+        k = 10
+        if st != ia.B_st:
+            for x in range(c.cnum[v]):
+                if t[k] != "*":
+                    assert abs(c.tsc[v][x] - float(t[k])) < 6e-4, (v, x, c.tsc[v][x], t[k])
+                k += 1
+        if st in (ia.ML_st, ia.MR_st, ia.IL_st, ia.IR_st):
+            for x in range(4):
+                assert abs(c.esc[v][x] - float(t[k + x])) < 6e-4
+                assert abs(c.oesc[v][x] - float(t[k + x])) < 6e-4
+            # degenerate N = average of the four scores (esl_abc_FAvgScore)
+            assert abs(c.oesc[v][15] - sum(c.esc[v][x] for x in range(4)) / 4) < 1e-6
+            assert c.oesc[v][4] < -9e35 and c.oesc[v][17] < -9e35
+        if st == ia.MP_st:
+            for x in range(16):
+                assert abs(c.esc[v][x] - float(t[k + x])) < 6e-4
+    # background renormalized
+    assert abs(sum(nullv) - 1.0) < 1e-6
+
+
+def test_integer_scores_round_float_scores():
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    c = cm.c
+    for v in range(c.M):
+        st = ord(c.sttype[v])
+        if st in (ia.B_st, ia.E_st):
+            continue
+        for x in range(c.cnum[v]):
+            f = c.tsc[v][x]
+            if f > -1e30:
+                assert abs(c.itsc[v][x] - f * 1000) <= 1.0
+        if c.beginsc[v] > -1e30:
+            assert abs(c.ibeginsc[v] - 1000 * c.beginsc[v]) <= 1.0
+        else:
+            assert c.ibeginsc[v] == -987654321
+
+
+def test_localize_closed_forms():
+    """cm_localize (src/cm_modelconfig.c:467-548): begin mass 1-pbegin on
+    node 1, pbegin spread evenly over internal MATP/MATL/MATR/BIF nodes; end
+    mass pend spread over exit nodes; root transitions zeroed."""
+    path = os.path.join(CMDIR, "Vault.c.cm")
+    g = ia.CM.read(path).configure(local=False)
+    l = ia.CM.read(path).configure(local=True)
+    cg, cl = g.c, l.c
+    ndtype = [ord(cl.ndtype[n]) for n in range(cl.nodes)]
+    nodemap = [cl.nodemap[n] for n in range(cl.nodes)]
+    starts = [n for n in range(2, cl.nodes) if ndtype[n] in (0, 1, 2, 3)]
+    exits = [n for n in range(1, cl.nodes) if ndtype[n] in (1, 2, 3, 4, 5) and n + 1 < cl.nodes and ndtype[n + 1] != 7]
+    assert abs(cl.begin[nodemap[1]] - 0.95) < 1e-7
+    for n in starts:
+        assert abs(cl.begin[nodemap[n]] - 0.05 / len(starts)) < 1e-9
+    assert abs(sum(cl.begin[v] for v in range(cl.M)) - 1.0) < 1e-5
+    exit_states = {nodemap[n] for n in exits}
+    for v in range(cl.M):
+        if v in exit_states:
+            assert abs(cl.end[v] - 0.05 / len(exits)) < 1e-9
+            tot = sum(cl.t[v][k] for k in range(cl.cnum[v])) + cl.end[v]
+            assert abs(tot - 1.0) < 1e-5
+            for k in range(cl.cnum[v]):
+                assert abs(cl.t[v][k] - cg.t[v][k] / (1 + cl.end[v])) < 1e-6
+        else:
+            assert cl.end[v] == 0.0
+    for k in range(cl.cnum[0]):
+        assert cl.tsc[0][k] < -9e35
+    assert cl.flags & ia.CMH_LOCAL_BEGIN and cl.flags & ia.CMH_LOCAL_END
+
+
+def test_logsum_tables_closed_form():
+    """FLogsum table entry i = log2(1 + 2^(-i/1000)); ILogsum table its x1000
+    rounding (src/logsum.c:74-148).  The reference evaluates these with the
+    truncated constants 1.44269504 and 0.69314718, so allow 1e-6 bits."""
+    lib = ia.lib()
+    for i in (0, 1, 2, 17, 999, 1000, 4321, 12345, 22999):
+        exact = math.log2(1 + 2 ** (-i / 1000))
+        a, b = 5.0, 5.0 - i / 1000.0
+        got = lib.FLogsum(a, b) - a
+        # index = (int)((max-min)*1000) may land one entry lower when the float
+        # difference rounds below i/1000
+        assert abs(got - exact) < 2e-3
+        assert abs(lib.ILogsum(5000, 5000 - i) - 5000 - round(1000 * exact)) <= 1
+    assert lib.FLogsum(3.0, 30.0) == 30.0          # difference >= 23: max
+    assert lib.FLogsum(-1e36, 2.0) == 2.0
+    assert abs(lib.FLogsum(1.5, 1.5) - 2.5) < 1e-6  # table[0] = 1
+    assert lib.ILogsum(-987654321, 100) == 100
+    assert lib.ILogsum(100, 100) == 1100
+
+
+def test_abi_exports_every_declared_function():
+    hdr = open(os.path.join(ROOT, "include", "infernal_gpu.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"^\s*(?:int|void|float|CM_TOPHITS \*)\s+\**([A-Za-z_]\w*)\s*\(", hdr, flags=re.M))
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", ia.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = sorted(n for n in names if n not in exported)
+    assert not missing, missing
+
+
+def test_contract_errors():
+    """Contract violations return eslEINCOMPAT before any device work
+    (src/cm_dpsearch.c:129-135, :1771-1776)."""
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    dsq = ia.dsq_from_codes(np.zeros(50, dtype=np.uint8))
+    with pytest.raises(ia.ScanError) as e:
+        ia.FastCYKScan(cm, 1, dsq, 10, 5, 0.0)
+    assert e.value.status == ia.eslEINCOMPAT
+    with pytest.raises(ia.ScanError) as e:
+        ia.FastCYKScan(cm, 3, dsq, 1, 50, 0.0)
+    assert e.value.status == ia.eslEINCOMPAT
+    # Inside scanners require CM_SEARCH_INSIDE; the wrapper sets it, so call the C ABI directly
+    import ctypes as C
+    lib = ia.lib()
+    cm.c.search_opts = 0
+    eb = C.create_string_buffer(128)
+    st = lib.FastFInsideScan(cm.ptr, eb, cm.c.smx, 1, dsq.ctypes.data, 1, 50, 0.0, None, 0, 0.0, None, None, None, None)
+    assert st == ia.eslEINCOMPAT and b"CM_SEARCH_INSIDE" in eb.value
+    bad = dsq.copy()
+    bad[5] = 17
+    with pytest.raises(ia.ScanError) as e:
+        ia.FastCYKScan(cm, 1, bad, 1, 50, 0.0)
+    assert e.value.status == ia.eslEINVAL
+
+
+@pytest.mark.skipif(ia.gpu_available(), reason="only meaningful without a GPU")
+def test_no_gpu_fails_loudly():
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    dsq = ia.dsq_from_codes(np.zeros(50, dtype=np.uint8))
+    with pytest.raises(ia.ScanError) as e:
+        ia.FastCYKScan(cm, 1, dsq, 1, 50, 0.0)
+    assert e.value.status == ia.eslENODEVICE
+
+
+def test_cells_per_residue_match_shipped_model_table():
+    """Cells and ops per residue at steady state for the shipped models
+    (QDB1, beta=1e-7): the figures measured independently from the .cm band
+    columns for the benchmark write-up."""
+    expect = {"tRNA.c.cm": (14629, 162506), "Vault.c.cm": (34899, 315066), "5S_rRNA.c.cm": (16134, 163376),
+              "Cobalamin.c.cm": (86217, 863089), "Plant_SRP.c.cm": (70370, 810985),
+              "rnaseP-eubact.c.cm": (83094, 841718)}
+    for name, (cells, ops) in expect.items():
+        cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=True)
+        c1, o1 = cm.cells_per_res(1)
+        assert (c1, o1) == (cells, ops), (name, c1, o1)
