@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: QDB-banded scanning CYK + Inside on one MI355X.
+
+Workload (the GPU FastCYKScan + FastFInsideScan configuration): a covariance
+model scanned over a seeded synthetic target of --mb megabases (second-order
+Markov background at 40% GC with --homologs emitted homolog fragments of
+0.5-1.0x model length planted at seeded positions).  One step = FastCYKScan
+with the tight bands (QDB1, beta 1e-7) over the whole target, then
+FastFInsideScan with the loose bands (QDB2, beta 1e-15) over the whole target
+("plain method": Inside on every position, no filter).
+
+value = DP cells of both scans / wall time of the step (Gcells/s), with the
+target already resident in device memory; cells are (v, j, d) cells inside
+the QDB bands, E states excluded, counted exactly as the reference scanners
+compute them.
+
+The JSON line also carries
+  roofline:     the dominant kernel's algorithmic VALU ops (2*cnum+1 per
+                non-B cell, 2 per bifurcation split, FLogsum counted as 8)
+                per second against MI355X's non-packed FP32 VALU peak, and
+  cpu_baseline: the C oracle (a restatement of the reference scanners,
+                gcc -O3, no FMA) timed on this host on a bounded sample.
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, non-packed FP32
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def ops_per_res(cm, qdbidx, inside):
+    """Algorithmic VALU ops per residue at steady state (bands at jp >= W)."""
+    cells, ops = cm.cells_per_res(qdbidx)
+    if not inside:
+        return ops
+    # each max becomes an FLogsum: ~8 VALU ops + 1 table read instead of 1 op;
+    # per child term 1 add + 1 logsum
+    return ops + (ops - cells) / 2.0 * 7.0
+
+
+def cpu_baseline(cm, codes, threads, sample_len, W):
+    """Oracle CYK (QDB1) + FInside (QDB2) on `threads` windows of sample_len+W-1."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import infernal_amd as ia
+    import oracle_bind as ob
+    ob.load()
+    Lw = sample_len + W - 1
+    rng = np.random.default_rng(99)
+    starts = rng.integers(1, len(codes) - Lw, threads)
+    dsq = ia.dsq_from_codes(codes)
+
+    def one(algo, q, s):
+        t = time.perf_counter()
+        ob.scan(cm, algo, q, dsq, int(s), int(s) + Lw - 1, 1e30, want_rows=False)
+        return time.perf_counter() - t
+
+    res = {}
+    for name, algo, q in (("cyk", ob.ORACLE_CYK, 1), ("finside", ob.ORACLE_FINSIDE, 2)):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda s: one(algo, q, s), starts))
+        wall = time.perf_counter() - t0
+        cells = 0.0
+        smx = cm.c.smx
+        for s in starts:
+            cells += _cells(cm, q, int(s), int(s) + Lw - 1)
+        res[name] = (cells, wall)
+    return res
+
+
+def _cells(cm, qdbidx, i0, j0):
+    import ctypes as C
+    import infernal_amd as ia
+    lib = ia.lib()
+    lib.infx_count_scan_cells.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
+    out = C.c_double()
+    lib.infx_count_scan_cells(C.cast(cm.ptr, C.c_void_p), cm.c.smx, qdbidx, 0, i0, j0, C.byref(out))
+    return out.value
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mb", type=float, default=100.0, help="target length in megabases")
+    ap.add_argument("--homologs", type=int, default=2000)
+    ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "rnaseP-eubact.c.cm"))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=3000, help="residues per CPU-baseline window")
+    ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (one wavefront per sequence chunk on a single MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import infernal_amd as ia
+    from infernal_amd import synth
+    if not ia.gpu_available():
+        print("bench.py: no usable gfx9 GPU", file=sys.stderr)
+        sys.exit(1)
+    if args.chunk:
+        ia.set_launch_params(args.chunk, 0)
+
+    cm = ia.CM.read(args.cm)
+    cm.configure(local=True)
+    n = int(args.mb * 1e6)
+    t = time.perf_counter()
+    bg = synth.markov2_background(n, gc=0.40, seed=20261203)
+    cm_g = ia.CM.read(args.cm)      # global model for emission
+    frags = synth.homologs(cm_g, args.homologs, seed=7) if args.homologs else []
+    codes, starts = synth.plant(bg, frags, seed=11) if frags else (bg, [])
+    dsq = ia.dsq_from_codes(codes)
+    log("target %.1f Mb, %d homologs, generated in %.1f s" % (n / 1e6, len(frags), time.perf_counter() - t))
+    M, W, clen = cm.M, cm.W, cm.clen
+    c1, _ = cm.cells_per_res(1)
+    c2, _ = cm.cells_per_res(2)
+    log("model %s clen %d M %d W %d; cells/res QDB1 %.0f QDB2 %.0f" % (cm.c.name.decode(), clen, M, W, c1, c2))
+
+    # warmup on a slice (first-use allocation, profile upload, code load)
+    wl = min(n, 2_000_000)
+    for _ in range(args.warmup):
+        if args.only in ("both", "cyk"):
+            ia.FastCYKScan(cm, 1, dsq, 1, wl, 0.0, want_hits=True)
+        if args.only in ("both", "finside"):
+            ia.FastFInsideScan(cm, 2, dsq, 1, wl, 0.0, want_hits=True)
+
+    step_times, cells_total, kern = [], 0.0, {"cyk": 0.0, "finside": 0.0}
+    cells_by = {"cyk": 0.0, "finside": 0.0}
+    hits = {}
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        if args.only in ("both", "cyk"):
+            r = ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
+            tm = ia.last_call_timing()
+            kern["cyk"] += tm["kernel_ms"]; cells_by["cyk"] += tm["cells"]
+            hits["cyk"] = len(r["hits"])
+            log("step %d CYK: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (s, tm["wall_ms"] / 1e3, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
+        if args.only in ("both", "finside"):
+            r = ia.FastFInsideScan(cm, 2, dsq, 1, n, 0.0, want_hits=True)
+            tm = ia.last_call_timing()
+            kern["finside"] += tm["kernel_ms"]; cells_by["finside"] += tm["cells"]
+            hits["finside"] = len(r["hits"])
+            log("step %d FInside: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (s, tm["wall_ms"] / 1e3, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
+        step_times.append(time.perf_counter() - t0)
+    total_t = sum(step_times)
+    cells_total = cells_by["cyk"] + cells_by["finside"]
+    value = cells_total / total_t / 1e9
+
+    # roofline of the dominant kernel
+    dom = max(kern, key=lambda k: kern[k])
+    q, inside = (1, False) if dom == "cyk" else (2, True)
+    ops = ops_per_res(cm, q, inside) * n * args.steps
+    achieved = ops / (kern[dom] / 1e3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": "std_scan_kernel<%s>" % dom,
+            "kernel_ms_per_launch": round(kern[dom] / args.steps, 1),
+            "gcells_per_s_kernel": {k: round(cells_by[k] / (kern[k] / 1e3) / 1e9, 2) for k in kern if kern[k] > 0}}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        r = cpu_baseline(cm, codes, threads, args.cpu_sample, W)
+        cc = r["cyk"][0] + r["finside"][0]
+        ct = r["cyk"][1] + r["finside"][1]
+        cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+               "sample": "oracle FastCYKScan (QDB1) + FastFInsideScan (QDB2), %d windows of %d nt in %d threads, "
+                         "%.1f s CYK + %.1f s FInside" % (threads, args.cpu_sample + W - 1, threads, r["cyk"][1], r["finside"][1]),
+               "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
+               "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4)}
+
+    out = {
+        "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU",
+        "value": round(value, 3),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_t / args.steps * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "FastCYKScan QDB1 + FastFInsideScan QDB2 over %.0f Mb (2nd-order Markov, 40%% GC, %d planted homologs), one strand"
+                               % (n / 1e6, len(frags)),
+                   "cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), clen, M, W),
+                   "cells_per_step": cells_total / args.steps, "hits": hits},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

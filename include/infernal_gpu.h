@@ -354,6 +354,21 @@ int cm_hit_AllowTruncation(CM_t *cm, int pass_idx, int64_t start, int64_t stop, 
  * the number of (v,d) cells inside the band at steady state (jp >= W). */
 int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells_per_res, double *ret_ops_per_res);
 
+/* Batched truncated scans (no counterpart in the reference, which makes one
+ * RefTrCYKScan / RefITrInsideScan call per window, src/cm_pipeline.c:4187-4195):
+ * nwin windows [wi0[w], wj0[w]] of one target dsq, each scanned exactly as an
+ * independent call (inside = 0: RefTrCYKScan, 1: RefITrInsideScan), in one
+ * launch.  Window w's hits go to hitlists[w] (if hitlists is non-NULL); its
+ * best score, mode and envelope to ret_sc[w], ret_mode[w], ret_envi[w],
+ * ret_envj[w] (each optional). */
+int infx_TrScanWindows(CM_t *cm, char *errbuf, CM_TR_SCAN_MX *trsmx, int inside, int qdbidx, int pass_idx, ESL_DSQ *dsq,
+                       int nwin, const int64_t *wi0, const int64_t *wj0, float cutoff, CM_TOPHITS **hitlists,
+                       int do_null3, float env_cutoff, float *ret_sc, char *ret_mode, int64_t *ret_envi, int64_t *ret_envj);
+
+/* Exact number of DP cells a scan of i0..j0 computes (E states excluded), from
+ * the band tables of <mx> (a CM_SCAN_MX, or a CM_TR_SCAN_MX if truncated). */
+int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells);
+
 /* ---------------------------------------------------------------------------
  * Library-level controls and introspection (no counterpart in the reference).
  */

@@ -1,0 +1,132 @@
+"""Seeded synthetic targets for the scanners: background sequence and
+homologs sampled from a covariance model.
+
+* markov2_background(n, gc, seed): iid-free background from a seeded
+  second-order Markov chain whose stationary GC content is `gc`.  Generated as
+  many independent chains of `seg` residues advanced together, so 100 Mb takes
+  seconds in numpy.
+* emit(cm, rng): one sequence sampled from the model's global parse-tree
+  distribution (transition and emission probabilities), as cmemit does with
+  EmitParsetree (src/cm_parsetree.c:1890): start at ROOT_S, emit by state
+  type, pick a child by t[v][], descend both branches of a bifurcation.
+* plant(background, homologs, seed): overwrite seeded, non-overlapping
+  positions of the background with the homolog fragments.
+"""
+import numpy as np
+
+from . import B_st, D_st, E_st, IL_st, IR_st, ML_st, MP_st, MR_st, S_st
+
+
+def markov2_background(n, gc=0.40, seed=1, seg=10000):
+    rng = np.random.default_rng(seed)
+    base = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    # per-context conditionals, then rescale them so the chain's stationary
+    # composition matches `base` (a few fixed-point rounds)
+    P = np.array([rng.dirichlet(base * 40.0) for _ in range(16)])
+    for _ in range(50):
+        T = np.zeros((16, 16))
+        for c in range(16):
+            for x in range(4):
+                T[c, (c % 4) * 4 + x] = P[c, x]
+        w, V = np.linalg.eig(T.T)
+        pi = np.real(V[:, np.argmin(np.abs(w - 1))])
+        pi = pi / pi.sum()
+        comp = np.zeros(4)
+        for c in range(16):
+            comp[c % 4] += pi[c]
+        P = P * (base / comp)[None, :]
+        P = P / P.sum(axis=1, keepdims=True)
+    cdf = np.cumsum(P, axis=1)
+    nseg = (n + seg - 1) // seg
+    out = np.empty((nseg, seg), dtype=np.uint8)
+    x1 = rng.choice(4, size=nseg, p=base)
+    x2 = rng.choice(4, size=nseg, p=base)
+    out[:, 0] = x1
+    out[:, 1] = x2
+    u = rng.random((seg, nseg), dtype=np.float32)
+    for t in range(2, seg):
+        ctx = out[:, t - 2].astype(np.int64) * 4 + out[:, t - 1]
+        out[:, t] = (u[t][:, None] > cdf[ctx]).sum(axis=1).clip(0, 3)
+    return out.reshape(-1)[:n].copy()
+
+
+def emit(cm, rng, max_len=100000):
+    """Sample one sequence (residue codes 0..3) from the model (global mode)."""
+    c = cm.c
+    M = c.M
+    st = [ord(c.sttype[v]) for v in range(M)]
+    cfirst = [c.cfirst[v] for v in range(M)]
+    cnum = [c.cnum[v] for v in range(M)]
+    t = [[c.t[v][k] for k in range(6)] for v in range(M)]
+    e = [[c.e[v][k] for k in range(16)] for v in range(M)]
+    root_t = [c.root_trans[k] for k in range(cnum[0])] if c.root_trans else t[0][:cnum[0]]
+
+    def pick(p):
+        p = np.asarray(p, dtype=np.float64)
+        s = p.sum()
+        return int(rng.choice(len(p), p=p / s))
+
+    # iterative walk; a node emits left residues on the way down and right
+    # residues on the way back up
+    left, right_stack = [], []
+    out = []
+    stack = [("v", 0)]
+    while stack:
+        kind, v = stack.pop()
+        if kind == "r":          # deferred right emission
+            out.append(v)
+            continue
+        s = st[v]
+        if s == E_st:
+            continue
+        if s == B_st:
+            stack.append(("v", cnum[v]))     # right subtree after the left one
+            stack.append(("v", cfirst[v]))
+            continue
+        if s in (ML_st, IL_st):
+            out.append(pick(e[v][:4]))
+        elif s in (MR_st, IR_st):
+            stack.append(("r", pick(e[v][:4])))
+        elif s == MP_st:
+            xy = pick(e[v][:16])
+            out.append(xy // 4)
+            stack.append(("r", xy % 4))
+        probs = root_t if v == 0 else t[v][:cnum[v]]
+        k = pick(probs)
+        stack.append(("v", cfirst[v] + k))
+        if len(out) > max_len:
+            break
+    return np.array(out, dtype=np.uint8)
+
+
+def homologs(cm, n, seed, frac=(0.5, 1.0)):
+    """n emitted sequences, each cut to a random fragment of `frac` of its length."""
+    rng = np.random.default_rng(seed)
+    res = []
+    for _ in range(n):
+        s = emit(cm, rng)
+        f = rng.uniform(*frac)
+        L = max(1, int(round(len(s) * f)))
+        a = int(rng.integers(0, len(s) - L + 1))
+        res.append(s[a:a + L])
+    return res
+
+
+def plant(background, frags, seed):
+    """Overwrite non-overlapping seeded positions; returns (codes, starts)."""
+    rng = np.random.default_rng(seed)
+    seq = background.copy()
+    n = len(seq)
+    total = sum(len(f) for f in frags)
+    if total >= n:
+        raise ValueError("homologs do not fit in the background")
+    # spread: pick sorted gap sizes that sum to n - total
+    gaps = rng.multinomial(n - total, np.ones(len(frags) + 1) / (len(frags) + 1))
+    starts = []
+    pos = 0
+    for g, f in zip(gaps[:-1], frags):
+        pos += int(g)
+        seq[pos:pos + len(f)] = f
+        starts.append(pos + 1)
+        pos += len(f)
+    return seq, starts

@@ -1,0 +1,88 @@
+"""GPU truncated scanners vs the oracle, through the C ABI.
+
+RefTrCYKScan bit-exact and RefITrInsideScan exact (integer) in every truncated
+pass (5' only, 3' only, 5' and 3' forced, 5' and 3' any), local and glocal:
+scores, best mode, hit lists (with modes), envelopes and per-state maxima.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import infernal_amd as ia
+import oracle_bind as ob
+from helpers import load_cm, random_dsq, hits_equal, describe_hit_diff
+
+pytestmark = pytest.mark.gpu
+
+PASSES = [ia.PLI_PASS_5P_ONLY_FORCE, ia.PLI_PASS_3P_ONLY_FORCE, ia.PLI_PASS_5P_AND_3P_FORCE, ia.PLI_PASS_5P_AND_3P_ANY]
+
+
+def _cmp(fn, algo, cm, qdb, p, dsq, i0, j0, cutoff, vsc=True, null3=False, greedy=True):
+    print("case", fn.__name__, cm.c.name.decode(), qdb, p, i0, j0, cutoff, null3, greedy, flush=True)
+    g = fn(cm, qdb, p, dsq, i0, j0, cutoff, do_null3=null3, env_cutoff=2.0, want_vsc=vsc, greedy=greedy)
+    o = ob.scan(cm, algo, qdb, dsq, i0, j0, cutoff, do_null3=null3, greedy=greedy, env_cutoff=2.0, pass_idx=p)
+    assert np.float32(g["sc"]).view(np.uint32) == np.float32(o["sc"]).view(np.uint32), (g["sc"], o["sc"])
+    assert g["mode"] == o["mode"], (g["mode"], o["mode"])
+    assert hits_equal(g["hits"], o["hits"]), describe_hit_diff(g["hits"], o["hits"])
+    assert (g["envi"], g["envj"]) == (o["envi"], o["envj"])
+    if vsc:
+        assert np.array_equal(g["vsc"].view(np.uint32), o["vsc"].view(np.uint32)), np.nonzero(g["vsc"] != o["vsc"])
+
+
+@pytest.mark.parametrize("p", PASSES)
+@pytest.mark.parametrize("local", [True, False], ids=["local", "glocal"])
+def test_trcyk_passes(p, local):
+    cm = load_cm("tRNA.c.cm", local=local, trunc=True)
+    dsq = random_dsq(400, seed=70 + p)
+    _cmp(ia.RefTrCYKScan, ob.ORACLE_TRCYK, cm, 1, p, dsq, 1, 400, -10.0)
+
+
+@pytest.mark.parametrize("p", PASSES)
+def test_triinside_passes(p):
+    cm = load_cm("tRNA.c.cm", local=True, trunc=True)
+    dsq = random_dsq(350, seed=80 + p)
+    _cmp(ia.RefITrInsideScan, ob.ORACLE_TRIINSIDE, cm, 2, p, dsq, 1, 350, -10.0)
+
+
+def test_trunc_bifurcating_and_options():
+    cm = load_cm("Vault.c.cm", local=True, trunc=True)
+    dsq = random_dsq(500, seed=91)
+    _cmp(ia.RefTrCYKScan, ob.ORACLE_TRCYK, cm, 1, ia.PLI_PASS_5P_AND_3P_FORCE, dsq, 21, 480, -5.0, null3=True)
+    _cmp(ia.RefTrCYKScan, ob.ORACLE_TRCYK, cm, 2, ia.PLI_PASS_5P_AND_3P_ANY, dsq, 1, 500, 0.0, greedy=False)
+    _cmp(ia.RefITrInsideScan, ob.ORACLE_TRIINSIDE, cm, 0, ia.PLI_PASS_3P_ONLY_FORCE, dsq, 1, 300, -5.0)
+    cm2 = load_cm("Cobalamin.c.cm", local=True, trunc=True)
+    dsq2 = random_dsq(600, seed=92)
+    _cmp(ia.RefTrCYKScan, ob.ORACLE_TRCYK, cm2, 1, ia.PLI_PASS_5P_AND_3P_FORCE, dsq2, 1, 600, -5.0)
+    _cmp(ia.RefITrInsideScan, ob.ORACLE_TRIINSIDE, cm2, 1, ia.PLI_PASS_5P_ONLY_FORCE, dsq2, 1, 600, -5.0)
+
+
+def test_trunc_window_batch():
+    """infx_TrScanWindows: many windows in one launch == one call per window."""
+    cm = load_cm("tRNA.c.cm", local=True, trunc=True)
+    dsq = random_dsq(3000, seed=93)
+    rng = np.random.default_rng(5)
+    n = 37
+    i0 = rng.integers(1, 2500, n).astype(np.int64)
+    j0 = (i0 + rng.integers(0, 450, n)).astype(np.int64)
+    lib = ia.lib()
+    lib.infx_TrScanWindows.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                       C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int, C.c_float,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    ths = [lib.cm_tophits_Create() for _ in range(n)]
+    arr = (C.c_void_p * n)(*[C.cast(t, C.c_void_p) for t in ths])
+    sc = np.zeros(n, np.float32); md = np.zeros(n, np.int8); ei = np.zeros(n, np.int64); ej = np.zeros(n, np.int64)
+    cm.c.search_opts &= ~ia.CM_SEARCH_INSIDE
+    eb = C.create_string_buffer(128)
+    st = lib.infx_TrScanWindows(C.cast(cm.ptr, C.c_void_p), eb, cm.c.trsmx, 0, 1, ia.PLI_PASS_5P_AND_3P_FORCE,
+                                dsq.ctypes.data, n, i0.ctypes.data, j0.ctypes.data, -5.0, arr, 0, 2.0,
+                                sc.ctypes.data, md.ctypes.data, ei.ctypes.data, ej.ctypes.data)
+    assert st == 0, eb.value
+    for w in range(n):
+        o = ob.scan(cm, ob.ORACLE_TRCYK, 1, dsq, int(i0[w]), int(j0[w]), -5.0, env_cutoff=2.0, pass_idx=ia.PLI_PASS_5P_AND_3P_FORCE)
+        got = ia._hits_list(ths[w])
+        assert np.float32(sc[w]).view(np.uint32) == np.float32(o["sc"]).view(np.uint32), (w, sc[w], o["sc"])
+        assert md[w] == o["mode"]
+        assert hits_equal(got, o["hits"]), (w, describe_hit_diff(got, o["hits"]))
+        assert (ei[w], ej[w]) == (o["envi"], o["envj"])
+        lib.cm_tophits_Destroy(ths[w])
