@@ -16,6 +16,8 @@
 #include "scan_host.h"
 
 extern "C" hipError_t infx_launch_std(int semi, int want_vsc, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a);
+extern "C" size_t infx_std_lds_bytes(int semi, int wpb, int acc_n);
+extern "C" int infx_std_occupancy(int semi, int want_vsc, int wpb, int acc_n);
 
 /* ------------------------------------------------------------------------ */
 /* scan matrices (src/cm_mx.c:5686-5836, :6391-6560)                          */
@@ -123,6 +125,7 @@ static char g_err[eslERRBUFSIZE];
 static hipStream_t g_stream;
 static hipEvent_t  g_ev0, g_ev1;
 static int  g_ncu = 256;
+static size_t g_lds_max = 65536;   /* LDS bytes one workgroup may allocate */
 static int  g_chunk_override = 0, g_wpb_override = 0;
 
 static thread_local double  t_wall_ms = 0, t_kernel_ms = 0;
@@ -148,6 +151,7 @@ extern "C" int infx_dev_init(char *errbuf)
     return infx_fail(eslENODEVICE, errbuf, "%s", g_err);
   }
   g_ncu = p.multiProcessorCount;
+  g_lds_max = p.sharedMemPerBlock;
   if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&g_ev0) != hipSuccess || hipEventCreate(&g_ev1) != hipSuccess) {
     snprintf(g_err, sizeof(g_err), "HIP stream/event creation failed");
@@ -178,7 +182,7 @@ void infx_timing_add(double wall, double kern, int64_t cells, int launches)
 extern "C" void infx_set_launch_params(int chunk_len, int waves_per_block)
 {
   g_chunk_override = chunk_len > 0 ? chunk_len : 0;
-  g_wpb_override   = (waves_per_block >= 1 && waves_per_block <= 4) ? waves_per_block : 0;
+  g_wpb_override   = (waves_per_block >= 1 && waves_per_block <= 16) ? waves_per_block : 0;
 }
 int infx_chunk_override(void) { return g_chunk_override; }
 int infx_wpb_override(void) { return g_wpb_override; }
@@ -327,25 +331,46 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
   if (semi == SEMI_IINSIDE) d0_init_int(cm, a0i); else d0_init_float(cm, a0f);
   const uint32_t impbits = (semi == SEMI_IINSIDE) ? (uint32_t) (-INFTY) : fbits((float) IMPOSSIBLE);
 
-  std::vector<int> base(M, 0);
+  /* BEGL decks: a ring per BEGL state, as many rows as the largest split
+   * k its B parent uses (+1), rounded up to a power of two */
+  std::vector<int> ring(M, 0);
+  for (int v = 1; v < M; v++) {
+    if (cm->sttype[v] != B_st || dn[v] > dx[v]) continue;
+    int w = cm->cfirst[v], maxk = 0;
+    for (int d = dn[v]; d <= dx[v]; d++) {
+      int kmin, kmax; kbounds(v, d, kmin, kmax);
+      if (kmin <= kmax) maxk = ESL_MAX(maxk, kmax);
+    }
+    int r = 1;
+    while (r < maxk + 1) r <<= 1;
+    ring[w] = r;
+  }
+
+  std::vector<int> base(M, 0), stride(M, 0);
   P.R = 0; P.RB = 0;
   P.rtmpl.clear(); P.dtmpl.clear();
   for (int v = 1; v < M; v++) {
     if (lo[v] > hi[v]) continue;
     lo[v] = ESL_MAX(lo[v], 0); hi[v] = ESL_MIN(hi[v], W);
     if (lo[v] > hi[v]) continue;
-    std::vector<uint32_t> &T = (cm->stid[v] == BEGL_S) ? P.dtmpl : P.rtmpl;
+    const bool begl = (cm->stid[v] == BEGL_S);
+    std::vector<uint32_t> &T = begl ? P.dtmpl : P.rtmpl;
     int off = (int) T.size();
     base[v] = off - lo[v];
-    for (int d = lo[v]; d <= hi[v]; d++) {
-      uint32_t b = impbits;
-      if (d == 0) b = (semi == SEMI_IINSIDE) ? (uint32_t) a0i[v] : fbits(a0f[v]);
-      T.push_back(b);
-    }
+    stride[v] = hi[v] - lo[v] + 1;
+    const int nrows = begl ? ESL_MAX(ring[v], 1) : 1;
+    if (begl) ring[v] = nrows;
+    for (int r = 0; r < nrows; r++)
+      for (int d = lo[v]; d <= hi[v]; d++) {
+        uint32_t b = impbits;
+        if (d == 0) b = (semi == SEMI_IINSIDE) ? (uint32_t) a0i[v] : fbits(a0f[v]);
+        T.push_back(b);
+      }
   }
   /* pad to a multiple of 64 words so every slot region stays aligned */
   while (P.rtmpl.size() % 64) P.rtmpl.push_back(impbits);
   while (P.dtmpl.size() % 64) P.dtmpl.push_back(impbits);
+  if (P.dtmpl.empty()) for (int i = 0; i < 64; i++) P.dtmpl.push_back(impbits);
   P.R = (int) P.rtmpl.size(); P.RB = (int) P.dtmpl.size();
 
   P.esc.clear();
@@ -382,10 +407,16 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
       s.endsc = fbits(cm->endsc[v]);
       if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = fbits(cm->tsc[v][k]);
     }
+    if (cm->stid[v] == BEGL_S) {
+      s.ring_mask = ring[v] - 1;
+      s.ring_stride = stride[v];
+    }
     if (st == B_st) {
       int w = cm->cfirst[v], y = cm->cnum[v];
       s.child_base[0] = base[w];
       s.child_base[1] = base[y];
+      s.ring_mask = ESL_MAX(ring[w], 1) - 1;
+      s.ring_stride = stride[w];
       s.banded = banded;
       if (banded) {
         s.kdn_y = ESL_MIN(dmin[y], smx->W); s.kdx_y = ESL_MIN(dmax[y], smx->W);
@@ -489,20 +520,40 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     if ((status = upload(D->d_lsum_i, infx_ilogsum_table(), LOGSUM_TBL * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
   }
 
-  /* work split: chunks of the target, >= 2W long so the W-1 warm-up stays small */
-  const int wpb = infx_wpb_override() ? infx_wpb_override() : 4;
+  /* launch shape: wavefronts per workgroup (LDS: log-sum table + per-wave
+   * local-begin accumulators), workgroups resident per CU */
+  const int acc_n = (P->root_dn <= P->root_dx) ? P->root_dx - P->root_dn + 1 : 0;
+  int wpb = infx_wpb_override() ? infx_wpb_override() : 16;
+  while (wpb > 1 && infx_std_lds_bytes(semi, wpb, acc_n) > (size_t) g_lds_max) wpb >>= 1;
+  if (infx_std_lds_bytes(semi, wpb, acc_n) > (size_t) g_lds_max)
+    return infx_fail(eslEINVAL, errbuf, "%s: W = %d needs more LDS than a workgroup has", fname, W);
+  const int occ = infx_std_occupancy(semi, ret_vsc != NULL, wpb, acc_n);
+  if (occ < 1) return infx_fail(eslESYS, errbuf, "%s: kernel cannot be made resident (wpb %d)", fname, wpb);
+  const int64_t resident = (int64_t) occ * wpb * g_ncu;
+
+  /* work split: one chunk per resident wavefront, each >= 2W long so the
+   * W-1 warm-up stays small */
   int64_t chunk = infx_chunk_override();
   if (chunk <= 0) {
-    int64_t target = (int64_t) g_ncu * 8;
-    chunk = (L + target - 1) / target;
+    chunk = (L + resident - 1) / resident;
     chunk = ESL_MAX(chunk, (int64_t) 2 * W);
     chunk = ESL_MAX(chunk, (int64_t) 64);
   }
   const int64_t nchunks64 = (L + chunk - 1) / chunk;
+  if (nchunks64 > INT32_MAX / 2) return infx_fail(eslERANGE, errbuf, "%s: too many chunks", fname);
   const int nchunks = (int) nchunks64;
-  const size_t slot_bytes = (((size_t) 2 * P->R + (size_t) (W + 1) * P->RB + 2 * (size_t) (W + 1) + (size_t) cm->M) * 4 + 255) / 256 * 256;
-  const size_t scratch_budget = (size_t) 24 << 30;
+  /* slot: guard | deck | guard | row0 | row1 | guard | vsc  (4-byte cells) */
+  const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
+  const size_t deck_off = G * 4;
+  const size_t row_off  = deck_off + ((size_t) P->RB + G) * 4;
+  const size_t vsc_off  = row_off + ((size_t) 2 * P->R + G) * 4;
+  const size_t slot_bytes = ((vsc_off + (size_t) cm->M * 4) + 255) / 256 * 256;
+  size_t mem_free = 0, mem_total = 0;
+  if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
+  const size_t scratch_budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
+                                + D->d_scratch.cap;
   int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
+  nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
 
   if ((status = D->d_dsq.reserve((size_t) L + 64)) != eslOK ||
@@ -527,6 +578,8 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.lsum = (semi == SEMI_FINSIDE) ? D->d_lsum_f.ptr : (semi == SEMI_IINSIDE) ? D->d_lsum_i.ptr : NULL;
   A.scratch = (char *) D->d_scratch.ptr;
   A.slot_bytes = (int64_t) slot_bytes;
+  A.deck_off = (int64_t) deck_off; A.row_off = (int64_t) row_off; A.vsc_off = (int64_t) vsc_off;
+  A.acc_n = acc_n;
   A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB;
   A.i0 = i0; A.j0 = j0;
   A.chunk_len = chunk; A.nchunks = nchunks; A.nslots = nslots;
@@ -585,7 +638,6 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   std::vector<float> vsc_slots;
   if (ret_vsc) {
     vsc_slots.resize((size_t) nslots * cm->M);
-    size_t vsc_off = ((size_t) 2 * P->R + (size_t) (W + 1) * P->RB + 2 * (size_t) (W + 1)) * 4;
     if (hipMemcpy2DAsync(vsc_slots.data(), (size_t) cm->M * 4, A.scratch + vsc_off, slot_bytes, (size_t) cm->M * 4, nslots,
                          hipMemcpyDeviceToHost, g_stream) != hipSuccess)
       return infx_fail(eslESYS, errbuf, "%s: vsc download failed", fname);

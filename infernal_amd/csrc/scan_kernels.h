@@ -1,17 +1,24 @@
-/* scan_kernels.h -- device-side data layout shared by the host launcher and
- * the gfx950 kernels of the standard (non-truncated) QDB scanners.
+/* scan_kernels.h -- device-side data layout shared by the host launchers and
+ * the gfx950 kernels of the QDB scanners.
  *
- * Storage of the DP values (one "slot" per resident wavefront):
- *   row[2][R]            the rolling j / j-1 alpha rows of every non-BEGL
- *                        state; state v holds cells d = lo_v..hi_v at
- *                        row[base_v + d]  (base_v = offset_v - lo_v)
- *   deck[W+1][RB]        the BEGL_S decks, ring over j (row j % (W+1))
- *   acc_sc[W+1], acc_r   per-d best local-begin candidate of the current j
+ * Standard scanners.  Device memory of one resident wavefront ("slot"):
+ *   guard[G]             G >= W + 128 cells that absorb the out-of-band deck
+ *                        reads of masked lanes in the bifurcation k-loop
+ *   deck[RB]             BEGL_S decks: state w keeps a ring of ring_w rows
+ *                        (a power of two >= the largest split k its B
+ *                        parent uses, + 1) of stride_w cells; the row of
+ *                        chunk position jp is (jp & (ring_w - 1)) and cell d
+ *                        is deck[base_w + row * stride_w + d]
+ *   guard[G]
+ *   row[2][R]            the rolling j / j-1 rows of every other state;
+ *                        state v holds cells d = lo_v..hi_v at row[base_v + d]
  *   vsc[M]               per-state running max (only when requested)
  * lo_v..hi_v covers the band and every cell a parent can read, so reads need
  * no bounds checks: cells outside the band hold their initial value
  * (IMPOSSIBLE, or the d = 0 value), exactly as the reference's full
  * alpha[2][M][W+1] would.
+ * LDS: per workgroup the log-sum table (Inside), per wavefront the local-begin
+ * accumulators acc_sc[acc_n], acc_r[acc_n] for d = root_dn .. root_dn + acc_n - 1.
  */
 #ifndef INFX_SCAN_KERNELS_H
 #define INFX_SCAN_KERNELS_H
@@ -23,10 +30,12 @@
 #define EMITPAIR  2
 #define EMITNONE  3
 
+#define INFX_LOGSUM_N 23000   /* entries of the FLogsum / ILogsum tables */
+
 enum { KIND_GENERIC = 0, KIND_IL = 1, KIND_B = 2, KIND_BEGL = 3, KIND_SKIP = 4 };
 enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };
 
-/* 128 bytes: two scalar-cache lines, read with uniform (SGPR) loads. */
+/* 128 bytes, read by the kernel with scalar loads */
 struct DState {
   int32_t kind;
   int32_t emitmode;
@@ -34,17 +43,18 @@ struct DState {
   int32_t rdelta;      /* children read from the j-1 row */
   int32_t cnum;        /* children (B: unused) */
   int32_t dn, dx;      /* band at steady state (jp >= W); per-j dx = min(jp, dx) */
-  int32_t base;        /* storage base (row, or deck row for BEGL) */
-  int32_t esc_off;     /* emission table offset, -1 if non-emitting */
+  int32_t base;        /* storage base (row, or deck ring for BEGL) */
+  int32_t esc_off;     /* emission table offset (0 if non-emitting) */
   int32_t has_begin;   /* finite local-begin score */
-  int32_t kdn_y, kdx_y, kdn_w, kdx_w;  /* B: min(dmin,W)/min(dmax,W) of BEGR y and BEGL w */
-  int32_t banded;      /* B: QDB k bounds (else 0..d) */
   int32_t init_ok;     /* endsc is finite: init(d) = el*d + endsc */
+  int32_t banded;      /* B: QDB k bounds (else 0..d) */
+  int32_t kdn_y, kdx_y, kdn_w, kdx_w;  /* B: min(dmin,W)/min(dmax,W) of BEGR y and BEGL w */
   uint32_t tsc[6];     /* float or int bits */
   uint32_t beginsc;    /* float or int bits */
   uint32_t endsc;      /* float or int bits */
   int32_t child_base[6];  /* storage base of child k (B: [0] = BEGL deck base, [1] = BEGR row base) */
-  int32_t pad[2];
+  int32_t ring_mask;   /* BEGL: its deck ring; B: the ring of its BEGL child */
+  int32_t ring_stride;
 };
 
 struct Cand {
@@ -63,10 +73,12 @@ struct ScanArgs {
   const void    *lsum;      /* log-sum table (Inside) */
   char          *scratch;
   int64_t        slot_bytes;
+  int64_t        deck_off, row_off, vsc_off;   /* byte offsets inside a slot */
   int            M, W, R, RB;
   int64_t        i0, j0;
   int64_t        chunk_len;
   int            nchunks, nslots;
+  int            acc_n;       /* LDS local-begin accumulators per wavefront */
   /* root state */
   int            root_dn, root_dx;
   int            root_cnum;

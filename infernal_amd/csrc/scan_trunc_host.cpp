@@ -282,7 +282,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
     hw[w].i0 = wi0[w]; hw[w].j0 = wj0[w]; hw[w].dsq_off = off - wi0[w];
   }
   const int W = trsmx->W;
-  const int wpb = infx_wpb_override() ? infx_wpb_override() : 4;
+  const int wpb = infx_wpb_override() ? ESL_MIN(infx_wpb_override(), 4) : 4;
   const size_t slot_bytes = (((size_t) 6 * P->R + 2 * (size_t) P->RT + 3 * (size_t) (W + 1) * P->RB + 8 * (size_t) (W + 1) + cm->M) * 4 + 255) / 256 * 256;
   const size_t budget = (size_t) 24 << 30;
   int nslots = (int) ESL_MIN((int64_t) nwin, (int64_t) (budget / slot_bytes));
