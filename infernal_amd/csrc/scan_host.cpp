@@ -15,9 +15,10 @@
 #include "scan_kernels.h"
 #include "scan_host.h"
 
-extern "C" hipError_t infx_launch_std(int semi, int want_vsc, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a);
-extern "C" size_t infx_std_lds_bytes(int semi, int wpb, int acc_n);
-extern "C" int infx_std_occupancy(int semi, int want_vsc, int wpb, int acc_n);
+extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, dim3 grid, dim3 block, hipStream_t stream,
+                                      const ScanArgs *a, int acc_n);
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n);
+extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n);
 
 /* ------------------------------------------------------------------------ */
 /* scan matrices (src/cm_mx.c:5686-5836, :6391-6560)                          */
@@ -216,11 +217,11 @@ struct StdProfile {
 
 struct SmxDev {
   std::vector<StdProfile *> prof;
-  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i;
+  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args;
   void  *trdev = NULL;   /* truncated-scanner resources */
   ~SmxDev() {
     for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
-    d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release();
+    d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release();
     d_lsum_f.release(); d_lsum_i.release();
     infx_trdev_release(trdev);
   }
@@ -367,64 +368,73 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
         T.push_back(b);
       }
   }
+  for (int v = 1; v < M; v++) if (cm->stid[v] == BEGL_S) ring[v] = ESL_MAX(ring[v], 1);
   /* pad to a multiple of 64 words so every slot region stays aligned */
   while (P.rtmpl.size() % 64) P.rtmpl.push_back(impbits);
   while (P.dtmpl.size() % 64) P.dtmpl.push_back(impbits);
   if (P.dtmpl.empty()) for (int i = 0; i < 64; i++) P.dtmpl.push_back(impbits);
   P.R = (int) P.rtmpl.size(); P.RB = (int) P.dtmpl.size();
 
+  /* state descriptors, v = M-1..1 without E states (packed, see scan_kernels.h) */
   P.esc.clear();
-  P.st.assign(M, DState());
-  for (int v = 0; v < M; v++) {
-    DState &s = P.st[v];
-    memset(&s, 0, sizeof(DState));
-    int st = cm->sttype[v];
-    s.kind = (st == E_st) ? KIND_SKIP : (st == B_st) ? KIND_B : (cm->stid[v] == BEGL_S) ? KIND_BEGL : (st == IL_st) ? KIND_IL : KIND_GENERIC;
-    if (v > 0 && s.kind != KIND_SKIP && cm->sttype[v] == IL_st && cm->cfirst[v] != v)
+  P.st.clear();
+  auto fits16 = [](long x) { return x >= 0 && x <= 0xffff; };
+  for (int v = M - 1; v >= 1; v--) {
+    const int st = cm->sttype[v];
+    if (st == E_st) continue;
+    if (st == IL_st && cm->cfirst[v] != v)
       return infx_fail(eslEINVAL, errbuf, "IL state %d does not loop on itself", v);
-    s.emitmode = Emitmode(st);
-    s.sd = StateDelta(st);
-    s.rdelta = StateRightDelta(st) > 0;
-    s.cnum = (st == B_st) ? 0 : cm->cnum[v];
-    s.dn = dn[v]; s.dx = dx[v];
+    DState s;
+    memset(&s, 0, sizeof(DState));
+    const int kind = (st == B_st) ? KIND_B : (cm->stid[v] == BEGL_S) ? KIND_BEGL : (st == IL_st) ? KIND_IL : KIND_GENERIC;
+    const int cnum = (st == B_st) ? 0 : cm->cnum[v];
+    int has_begin, init_ok;
+    if (semi == SEMI_IINSIDE) { has_begin = cm->ibeginsc[v] != -INFTY; init_ok = cm->iendsc[v] != -INFTY; }
+    else                      { has_begin = NOT_IMPOSSIBLE(cm->beginsc[v]); init_ok = NOT_IMPOSSIBLE(cm->endsc[v]); }
+    if (v > 0xffff || cnum > 7 || !fits16(dn[v]) || !fits16(dx[v]))
+      return infx_fail(eslERANGE, errbuf, "state %d does not fit the packed scan descriptor (M, cnum or band too large)", v);
+    s.hdr = (uint32_t) kind | ((uint32_t) Emitmode(st) << 3) | ((uint32_t) StateDelta(st) << 5) |
+            ((uint32_t) (StateRightDelta(st) > 0) << 7) | ((uint32_t) cnum << 8) | ((uint32_t) has_begin << 11) |
+            ((uint32_t) init_ok << 12) | ((uint32_t) banded << 13) | ((uint32_t) v << 16);
+    s.dnx = (uint32_t) dn[v] | ((uint32_t) dx[v] << 16);
     s.base = base[v];
-    s.esc_off = 0;
     if (st == ML_st || st == MR_st || st == IL_st || st == IR_st || st == MP_st) {
       int n = (st == MP_st) ? INFX_KP * INFX_KP : INFX_KP;
-      s.esc_off = (int) P.esc.size();
+      s.aux = (uint32_t) P.esc.size();
       for (int x = 0; x < n; x++) P.esc.push_back(semi == SEMI_IINSIDE ? (uint32_t) cm->ioesc[v][x] : fbits(cm->oesc[v][x]));
     }
-    if (semi == SEMI_IINSIDE) {
-      s.has_begin = cm->ibeginsc[v] != -INFTY;
-      s.beginsc = (uint32_t) cm->ibeginsc[v];
-      s.init_ok = cm->iendsc[v] != -INFTY;
-      s.endsc = (uint32_t) cm->iendsc[v];
-      if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = (uint32_t) cm->itsc[v][k];
-    } else {
-      s.has_begin = NOT_IMPOSSIBLE(cm->beginsc[v]);
-      s.beginsc = fbits(cm->beginsc[v]);
-      s.init_ok = NOT_IMPOSSIBLE(cm->endsc[v]);
-      s.endsc = fbits(cm->endsc[v]);
-      if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = fbits(cm->tsc[v][k]);
-    }
+    if (semi == SEMI_IINSIDE) { s.beginsc = (uint32_t) cm->ibeginsc[v]; s.endsc = (uint32_t) cm->iendsc[v]; }
+    else                      { s.beginsc = fbits(cm->beginsc[v]);       s.endsc = fbits(cm->endsc[v]); }
     if (cm->stid[v] == BEGL_S) {
-      s.ring_mask = ring[v] - 1;
-      s.ring_stride = stride[v];
+      if (!fits16(ring[v] - 1) || !fits16(stride[v])) return infx_fail(eslERANGE, errbuf, "BEGL deck of state %d too large", v);
+      s.aux = (uint32_t) (ring[v] - 1) | ((uint32_t) stride[v] << 16);
     }
     if (st == B_st) {
       int w = cm->cfirst[v], y = cm->cnum[v];
-      s.child_base[0] = base[w];
-      s.child_base[1] = base[y];
-      s.ring_mask = ESL_MAX(ring[w], 1) - 1;
-      s.ring_stride = stride[w];
-      s.banded = banded;
+      int kdn_y = 0, kdx_y = 0, kdn_w = 0, kdx_w = 0;
       if (banded) {
-        s.kdn_y = ESL_MIN(dmin[y], smx->W); s.kdx_y = ESL_MIN(dmax[y], smx->W);
-        s.kdn_w = ESL_MIN(dmin[w], smx->W); s.kdx_w = ESL_MIN(dmax[w], smx->W);
+        kdn_y = ESL_MIN(dmin[y], smx->W); kdx_y = ESL_MIN(dmax[y], smx->W);
+        kdn_w = ESL_MIN(dmin[w], smx->W); kdx_w = ESL_MIN(dmax[w], smx->W);
       }
-    } else if (st != E_st) {
-      for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.child_base[k] = base[cm->cfirst[v] + k];
+      if (!fits16(kdn_y) || !fits16(kdx_y) || !fits16(kdn_w) || !fits16(kdx_w) ||
+          !fits16(ESL_MAX(ring[w], 1) - 1) || !fits16(stride[w]))
+        return infx_fail(eslERANGE, errbuf, "B state %d does not fit the packed scan descriptor", v);
+      s.tsc[0] = (uint32_t) kdn_y | ((uint32_t) kdx_y << 16);
+      s.tsc[1] = (uint32_t) kdn_w | ((uint32_t) kdx_w << 16);
+      s.tsc[2] = (uint32_t) base[w];
+      s.tsc[3] = (uint32_t) base[y];
+      s.aux = (uint32_t) (ESL_MAX(ring[w], 1) - 1) | ((uint32_t) stride[w] << 16);
+    } else {
+      for (int k = 0; k < cnum && k < 6; k++)
+        s.tsc[k] = (semi == SEMI_IINSIDE) ? (uint32_t) cm->itsc[v][k] : fbits(cm->tsc[v][k]);
+      s.cbase = base[cm->cfirst[v]];
+      for (int k = 0; k < cnum && k < 6; k++) {
+        long off = (long) base[cm->cfirst[v] + k] - s.cbase;
+        if (off < -32768 || off > 32767) return infx_fail(eslERANGE, errbuf, "children of state %d are too far apart for the packed scan descriptor", v);
+        s.coff[k >> 1] |= ((uint32_t) (uint16_t) (int16_t) off) << (16 * (k & 1));
+      }
     }
+    P.st.push_back(s);
   }
   P.root_dn = dn[0]; P.root_dx = dx[0];
   P.root_cnum = cm->cnum[0];
@@ -486,16 +496,16 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   if (j0 - i0 + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
   for (int64_t x = i0; x <= j0; x++)
     if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
-  if ((status = infx_dev_init(errbuf)) != eslOK) return status;
-
   const int64_t L = j0 - i0 + 1;
   const int W = (smx->W > L) ? (int) L : smx->W;
-  if (!smx->dev) smx->dev = new SmxDev();
-  SmxDev *D = (SmxDev *) smx->dev;
 
-  /* profile */
+  /* profile (host only: its errors are reported with or without a device) */
   StdProfile fresh;
   if ((status = build_std_profile(cm, smx, qdbidx, semi, W, fresh, errbuf)) != eslOK) return status;
+
+  if ((status = infx_dev_init(errbuf)) != eslOK) return status;
+  if (!smx->dev) smx->dev = new SmxDev();
+  SmxDev *D = (SmxDev *) smx->dev;
   StdProfile *P = NULL;
   for (auto *p : D->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->W == W) P = p;
   if (!P) { P = new StdProfile(); D->prof.push_back(P); }
@@ -523,12 +533,18 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   /* launch shape: wavefronts per workgroup (LDS: log-sum table + per-wave
    * local-begin accumulators), workgroups resident per CU */
   const int acc_n = (P->root_dn <= P->root_dx) ? P->root_dx - P->root_dn + 1 : 0;
-  int wpb = infx_wpb_override() ? infx_wpb_override() : 16;
-  while (wpb > 1 && infx_std_lds_bytes(semi, wpb, acc_n) > (size_t) g_lds_max) wpb >>= 1;
-  if (infx_std_lds_bytes(semi, wpb, acc_n) > (size_t) g_lds_max)
-    return infx_fail(eslEINVAL, errbuf, "%s: W = %d needs more LDS than a workgroup has", fname, W);
-  const int occ = infx_std_occupancy(semi, ret_vsc != NULL, wpb, acc_n);
-  if (occ < 1) return infx_fail(eslESYS, errbuf, "%s: kernel cannot be made resident (wpb %d)", fname, wpb);
+  /* the workgroup size that keeps the most wavefronts resident (ties: larger) */
+  int tbl_lds = 1;
+  if (const char *e = getenv("INFX_LOGSUM_TABLE")) tbl_lds = strcmp(e, "l2") != 0;
+  int wpb = 0, occ = 0;
+  for (int w = 16; w >= 1; w >>= 1) {
+    if (infx_wpb_override() && w != infx_wpb_override()) continue;
+    if (infx_std_lds_bytes(semi, tbl_lds, w, acc_n) > (size_t) g_lds_max) continue;
+    const int o = infx_std_occupancy(semi, ret_vsc != NULL, tbl_lds, w, acc_n);
+    if (o * w > occ * wpb) { wpb = w; occ = o; }
+  }
+  if (wpb < 1 || occ < 1)
+    return infx_fail(eslEINVAL, errbuf, "%s: no workgroup shape fits (W = %d, LDS limit %zu bytes)", fname, W, (size_t) g_lds_max);
   const int64_t resident = (int64_t) occ * wpb * g_ncu;
 
   /* work split: one chunk per resident wavefront, each >= 2W long so the
@@ -570,6 +586,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   ScanArgs A;
   memset(&A, 0, sizeof(A));
   A.st = (const DState *) P->d_st.ptr;
+  A.nst = (int) P->st.size();
   A.esc = P->d_esc.ptr;
   A.dsq = (const uint8_t *) D->d_dsq.ptr;
   A.dsq_off = i0;
@@ -606,13 +623,17 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.chunk_envj = A.chunk_envi + nchunks;
 
   dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
+  if (getenv("INFX_VERBOSE"))
+    fprintf(stderr, "[infx] %s: L %lld W %d chunk %lld nchunks %d nslots %d wpb %d blocks/CU %d slot %zu B table %s\n", fname,
+            (long long) L, W, (long long) chunk, nchunks, nslots, wpb, occ, slot_bytes, tbl_lds ? "lds" : "l2");
   unsigned long long ncand = 0;
   float kernel_ms_total = 0;
   int launches = 0;
   for (int attempt = 0; attempt < 3; attempt++) {
     if (hipMemsetAsync(A.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
     hipEventRecord(g_ev0, g_stream);
-    hipError_t e = infx_launch_std(semi, A.want_vsc, grid, block, g_stream, &A);
+    if (upload(D->d_args, &A, sizeof(A)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
+    hipError_t e = infx_launch_std(semi, A.want_vsc, tbl_lds, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_n);
     hipEventRecord(g_ev1, g_stream);
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, A.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||

@@ -35,26 +35,22 @@
 enum { KIND_GENERIC = 0, KIND_IL = 1, KIND_B = 2, KIND_BEGL = 3, KIND_SKIP = 4 };
 enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };
 
-/* 128 bytes, read by the kernel with scalar loads */
+/* 64 bytes: one scalar load.  hdr bits: kind 0-2, emitmode 3-4, sd 5-6,
+ * rdelta 7 (children read from the j-1 row), cnum 8-10, has_begin 11,
+ * init_ok 12 (endsc finite: init(d) = el*d + endsc), banded 13 (B: QDB k
+ * bounds, else 0..d), state index v 16-31. */
 struct DState {
-  int32_t kind;
-  int32_t emitmode;
-  int32_t sd;          /* StateDelta */
-  int32_t rdelta;      /* children read from the j-1 row */
-  int32_t cnum;        /* children (B: unused) */
-  int32_t dn, dx;      /* band at steady state (jp >= W); per-j dx = min(jp, dx) */
-  int32_t base;        /* storage base (row, or deck ring for BEGL) */
-  int32_t esc_off;     /* emission table offset (0 if non-emitting) */
-  int32_t has_begin;   /* finite local-begin score */
-  int32_t init_ok;     /* endsc is finite: init(d) = el*d + endsc */
-  int32_t banded;      /* B: QDB k bounds (else 0..d) */
-  int32_t kdn_y, kdx_y, kdn_w, kdx_w;  /* B: min(dmin,W)/min(dmax,W) of BEGR y and BEGL w */
-  uint32_t tsc[6];     /* float or int bits */
+  uint32_t hdr;
+  uint32_t dnx;        /* dn | dx << 16: band at steady state; per-j dx = min(jp, dx) */
+  int32_t  base;       /* storage base (row, or deck ring for BEGL) */
+  uint32_t aux;        /* emitting: esc_off; BEGL, B: ring_mask | ring_stride << 16 (B: of its BEGL) */
   uint32_t beginsc;    /* float or int bits */
   uint32_t endsc;      /* float or int bits */
-  int32_t child_base[6];  /* storage base of child k (B: [0] = BEGL deck base, [1] = BEGR row base) */
-  int32_t ring_mask;   /* BEGL: its deck ring; B: the ring of its BEGL child */
-  int32_t ring_stride;
+  uint32_t tsc[6];     /* float or int bits.  B: [0] kdn_y | kdx_y << 16, [1] kdn_w | kdx_w << 16
+                          (min(dmin,W) / min(dmax,W) of BEGR y and BEGL w), [2] BEGL deck base,
+                          [3] BEGR row base */
+  int32_t  cbase;      /* child k storage base = cbase + coff_k */
+  uint32_t coff[3];    /* int16 coff_k: low half (k even) / high half (k odd) of coff[k/2] */
 };
 
 struct Cand {
@@ -63,8 +59,10 @@ struct Cand {
   int32_t r;
 };
 
+/* launch parameters; the kernel reads them from device memory */
 struct ScanArgs {
-  const DState  *st;        /* [M] */
+  const DState  *st;        /* [nst]: every state but E and ROOT_S, v descending */
+  int            nst;
   const void    *esc;       /* emission scores, float or int */
   const uint8_t *dsq;       /* dsq[x - dsq_off] for x in [i0, j0] */
   int64_t        dsq_off;

@@ -52,17 +52,29 @@
 #define WPB_MAX     16
 
 typedef const __attribute__((address_space(4))) uint32_t CU32;
+typedef const __attribute__((address_space(4))) ScanArgs CArgs;
 
-/* state descriptor through the scalar cache (uniform address, read-only) */
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+typedef const __attribute__((address_space(4))) u32x16 CU32x16;
+
+/* state descriptor through the scalar cache: one s_load_dwordx16 */
 __device__ __forceinline__ DState load_state(const DState *p)
 {
+  static_assert(sizeof(DState) == 64, "DState is one 64-byte scalar load");
+  const u32x16 w = *(CU32x16 *) p;
   DState s;
-  uint32_t *dst = (uint32_t *) &s;
-  CU32 *src = (CU32 *) p;
-#pragma unroll
-  for (int i = 0; i < (int) (sizeof(DState) / 4); i++) dst[i] = src[i];
+  s.hdr = w[0]; s.dnx = w[1]; s.base = (int32_t) w[2]; s.aux = w[3];
+  s.beginsc = w[4]; s.endsc = w[5];
+  s.tsc[0] = w[6]; s.tsc[1] = w[7]; s.tsc[2] = w[8]; s.tsc[3] = w[9]; s.tsc[4] = w[10]; s.tsc[5] = w[11];
+  s.cbase = (int32_t) w[12]; s.coff[0] = w[13]; s.coff[1] = w[14]; s.coff[2] = w[15];
   return s;
 }
+
+/* The launch parameters are read through a pointer the compiler cannot
+ * hoist loads from: each use is a fresh scalar load (scalar-cache hit), so
+ * the parameters of the root and hit-reporting steps hold no SGPRs during
+ * the state sweep. */
+__device__ __forceinline__ CArgs *fresh(CArgs *p) { asm volatile("" : "+s"(p)); return p; }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -189,40 +201,6 @@ __device__ __forceinline__ typename SR<S>::T combine_children(const typename SR<
   return sc;
 }
 
-/* One 64-cell block of a non-B state with CN children: the combined child
- * terms of cell d (IL: without the self term).  BEGL decks combine
- * init, c0, c1, ... in child order (src/cm_dpsearch.c:430-444). */
-template <int S, int CN, bool BEGL, bool IL>
-__device__ __forceinline__ typename SR<S>::T block_cells(const typename SR<S>::T *crow, const int32_t *cb, const uint32_t *tsc,
-                                                         int dy, typename SR<S>::T ini, const typename SR<S>::T *tbl)
-{
-  typedef SR<S> R;
-  typename R::T c[6];
-#pragma unroll
-  for (int k = 0; k < CN; k++) c[k] = (IL && k == 0) ? R::imp() : R::add(crow[cb[k] + dy], R::bits(tsc[k]));
-  if (BEGL) {
-    typename R::T sc = ini;
-#pragma unroll
-    for (int k = 0; k < CN; k++) sc = R::op(sc, c[k], tbl);
-    return sc;
-  }
-  return combine_children<S, CN>(c, ini, !IL, tbl);
-}
-
-template <int S, bool BEGL, bool IL>
-__device__ __forceinline__ typename SR<S>::T block_dispatch(int cnum, const typename SR<S>::T *crow, const int32_t *cb, const uint32_t *tsc,
-                                                            int dy, typename SR<S>::T ini, const typename SR<S>::T *tbl)
-{
-  switch (cnum) {
-  case 1:  return block_cells<S, 1, true, false>(crow, cb, tsc, dy, ini, tbl);  /* BEGL only: other states have 2..6 children */
-  case 2:  return block_cells<S, 2, BEGL, IL>(crow, cb, tsc, dy, ini, tbl);
-  case 3:  return block_cells<S, 3, BEGL, IL>(crow, cb, tsc, dy, ini, tbl);
-  case 4:  return block_cells<S, 4, BEGL, IL>(crow, cb, tsc, dy, ini, tbl);
-  case 5:  return block_cells<S, 5, BEGL, IL>(crow, cb, tsc, dy, ini, tbl);
-  default: return block_cells<S, 6, BEGL, IL>(crow, cb, tsc, dy, ini, tbl);
-  }
-}
-
 /* emission score of residue code x from a register whose lane y holds the
  * score of code y (ds_bpermute crossbar, no LDS storage) */
 template <int S>
@@ -231,57 +209,238 @@ __device__ __forceinline__ typename SR<S>::T pick_esc(int ev_bits, int x)
   return SR<S>::fromb(__builtin_amdgcn_ds_bpermute(x << 2, ev_bits));
 }
 
+/* Per-j context shared by the state loops below, passed as plain values
+ * (a struct here would be selected by field address and spilled). */
+#define CTX_PARAMS const typename SR<S>::T *cur_, const typename SR<S>::T *prv_, typename SR<S>::T *acc_sc, int *acc_r, \
+                   const typename SR<S>::T *tbl, const uint8_t *dsq_b, int dsq_w, int xj, int jp, int rdn, uint32_t el, int lane
+#define CTX_ARGS   cur, prv, acc_sc, acc_r, tbl, dsq_b, dsq_w, xj, jp, rdn, el, lane
+
+/* local-begin candidate of cell d (src/cm_dpsearch.c:549-560): the last
+ * state with the largest alpha + beginsc wins ties (v descends) */
+template <int S>
+__device__ __forceinline__ void begin_update(typename SR<S>::T *acc_sc, int *acc_r, int rdn, bool on, int d,
+                                             typename SR<S>::T sc, typename SR<S>::T bsc, int v)
+{
+  typedef SR<S> R;
+  if (on) {
+    const typename R::T cand = R::add(sc, bsc);
+    if (cand >= acc_sc[d - rdn]) { acc_sc[d - rdn] = cand; acc_r[d - rdn] = v; }
+  }
+}
+
+/* A non-B, non-IL state with CN children over its band [dn, dx] (64 cells
+ * per block).  BEGL decks combine init, c0, c1, ... in child order
+ * (src/cm_dpsearch.c:430-444); other states use combine_children. */
+template <int S, bool VSC, int CN, bool BEGL>
+__device__ __forceinline__ float gen_state(CTX_PARAMS, const DState &s, typename SR<S>::T *out, int dn, int dx,
+                                           bool has_begin, int bdn, int bdx, int v, int ev)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
+  const uint32_t hdr = s.hdr;
+  const int em = (hdr >> 3) & 3, sd = (hdr >> 5) & 3;
+  const T *crow = ((hdr >> 7) & 1) ? prv_ : cur_;
+  const int init_ok = (hdr >> 12) & 1;
+  const T bsc = R::bits(s.beginsc);
+  int cb[CN];
+#pragma unroll
+  for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
+  const T eright = R::fromb(__builtin_amdgcn_readlane(ev, xj));
+  float vmax = IMPF;
+  for (int d0 = dn; d0 <= dx; d0 += 64) {
+    const int d = d0 + lane;
+    const bool act = d <= dx;
+    const int dd = act ? d : dx;
+    const int dy = dd - sd;
+    const int xl = (int) dsq_b[(uint32_t) (dsq_w - dd)];
+    T ch[CN];
+#pragma unroll
+    for (int k = 0; k < CN; k++) ch[k] = R::add(crow[(uint32_t) (cb[k] + dy)], R::bits(s.tsc[k]));
+    const T ini = R::init(init_ok, el, s.endsc, dy);
+    T sc;
+    if (BEGL) {
+      sc = ini;
+#pragma unroll
+      for (int k = 0; k < CN; k++) sc = R::op(sc, ch[k], tbl);
+    } else {
+      sc = combine_children<S, CN>(ch, ini, true, tbl);
+      const T e = (em == EMITRIGHT) ? eright : pick_esc<S>(ev, xl);
+      if (em != EMITNONE) sc = R::add(sc, e);
+    }
+    if (act) {
+      out[(uint32_t) (s.base + d)] = sc;
+      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, sc, bsc, v);
+      if (VSC) vmax = fmaxf(vmax, R::score(sc));
+    }
+  }
+  return vmax;
+}
+
+/* IL state: x_d = op(pre_d, x_{d-1} + t0) + e_d with pre_d the other CN-1
+ * child terms combined with init (the self term is applied last), solved
+ * per 64-cell block by the exact fixed-point sweep. */
+template <int S, bool VSC, int CN>
+__device__ __forceinline__ float il_state(CTX_PARAMS, const DState &s, typename SR<S>::T *cur, int dn, int dx,
+                                          bool has_begin, int bdn, int bdx, int v, int ev)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
+  const int init_ok = (s.hdr >> 12) & 1;
+  const T bsc = R::bits(s.beginsc);
+  const T t0 = R::bits(s.tsc[0]);
+  int cb[CN];
+#pragma unroll
+  for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
+  T carry = cur[(uint32_t) (s.base + dn - 1)];
+  float vmax = IMPF;
+  for (int d0 = dn; d0 <= dx; d0 += 64) {
+    const int d = d0 + lane;
+    const bool act = d <= dx;
+    const int dd = act ? d : dx;            /* inactive lanes mirror a valid cell */
+    const int dy = dd - 1;
+    const int xl = (int) dsq_b[(uint32_t) (dsq_w - dd)];
+    T ch[CN];
+    ch[0] = R::imp();
+#pragma unroll
+    for (int k = 1; k < CN; k++) ch[k] = R::add(cur[(uint32_t) (cb[k] + dy)], R::bits(s.tsc[k]));
+    const T ini = R::init(init_ok, el, s.endsc, dy);
+    const T pre = combine_children<S, CN>(ch, ini, false, tbl);
+    const T e   = pick_esc<S>(ev, xl);
+    const int cbits = R::tob(carry);
+    T x = R::add(R::op(pre, R::add(lane == 0 ? carry : R::imp(), t0), tbl), e);
+    for (int it = 0; it < 65; it++) {
+      const T left = R::fromb(shr1(cbits, R::tob(x)));
+      const T xn = R::add(R::op(pre, R::add(left, t0), tbl), e);
+      const bool changed = act && !R::same(xn, x);
+      x = xn;
+      if (!__any(changed)) break;
+    }
+    if (act) {
+      cur[(uint32_t) (s.base + d)] = x;
+      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, x, bsc, v);
+      if (VSC) vmax = fmaxf(vmax, R::score(x));
+    }
+    carry = R::fromb(__builtin_amdgcn_readlane(R::tob(x), 63));
+  }
+  return vmax;
+}
+
+/* B state (src/cm_dpsearch.c:246-284): all lanes walk the union of their
+ * k ranges together, so one k reads one BEGL deck row (consecutive columns)
+ * and one BEGR cell (v_readlane broadcast); each lane keeps its own
+ * kmin(d)..kmax(d) in ascending order. */
+template <int S, bool VSC>
+__device__ __forceinline__ float b_state(CTX_PARAMS, const DState &s, typename SR<S>::T *cur, typename SR<S>::T *deck,
+                                         int dn, int dx, bool has_begin, int bdn, int bdx, int v)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
+  const uint32_t hdr = s.hdr;
+  const int init_ok = (hdr >> 12) & 1;
+  const int banded = (hdr >> 13) & 1;
+  const T bsc = R::bits(s.beginsc);
+  const int kdn_y = s.tsc[0] & 0xffff, kdx_y = s.tsc[0] >> 16;
+  const int kdn_w = s.tsc[1] & 0xffff, kdx_w = s.tsc[1] >> 16;
+  const int by = (int) s.tsc[3];
+  const int rmask = s.aux & 0xffff, rstride = s.aux >> 16;
+  const T *wbase = deck + (int) s.tsc[2];
+  float vmax = IMPF;
+  for (int d0 = dn; d0 <= dx; d0 += 64) {
+    const int d = d0 + lane;
+    const bool act = d <= dx;
+    const int dd = act ? d : dx;
+    const int dl = min(d0 + 63, dx);
+    int kmin, kmax, K0, K1;
+    if (banded) {
+      kmin = max(0, max(kdn_y, dd - kdx_w));
+      kmax = min(kdx_y, dd - kdn_w);
+      K0 = max(0, max(kdn_y, d0 - kdx_w));   /* both bounds grow with d */
+      K1 = min(kdx_y, dl - kdn_w);
+    } else { kmin = 0; kmax = dd; K0 = 0; K1 = dl; }
+    T sc = R::init(init_ok, el, s.endsc, dd);
+    for (int kb = K0; kb <= K1; kb += 64) {
+      const int ke = min(K1, kb + 63);
+      const int yb = R::tob(cur[(uint32_t) (by + min(kb + lane, ke))]);
+      /* four k per step, loads first; k > ke is masked (its reads stay
+       * inside the slot's guard cells) */
+      for (int k = kb; k <= ke; k += 4) {
+        T bl[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int kk = k + u;
+          yv[u] = R::fromb(__builtin_amdgcn_readlane(yb, min(kk - kb, 63)));
+          const T *rowp = wbase + (((jp - kk) & rmask) * rstride - kk);
+          bl[u] = rowp[(uint32_t) dd];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int kk = k + u;
+          const T nsc = R::op(sc, R::add(bl[u], yv[u]), tbl);
+          sc = (kk <= ke && kk >= kmin && kk <= kmax) ? nsc : sc;
+        }
+      }
+    }
+    if (act) {
+      cur[(uint32_t) (s.base + d)] = sc;
+      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, sc, bsc, v);
+      if (VSC) vmax = fmaxf(vmax, R::score(sc));
+    }
+  }
+  return vmax;
+}
+
 /* ------------------------------------------------------------------------ */
 
-template <int S, bool VSC>
-__global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
+template <int S, bool VSC, bool TBL_LDS>
+__device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 {
   typedef SR<S> R;
   typedef typename R::T T;
   extern __shared__ __align__(16) char lds_raw[];
+  CArgs *ap = (CArgs *) ap_;
 
   const int lane = threadIdx.x & 63;
-  const int wib  = threadIdx.x >> 6;
+  const int wib  = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   /* wave-uniform: slot pointers live in SGPRs */
   const int slot = blockIdx.x * (blockDim.x >> 6) + wib;
 
-  /* log-sum table into LDS (Inside) */
-  const T *tbl = (const T *) a.lsum;
+  /* log-sum table (Inside): staged in LDS, or read through L1/L2 */
+  const T *tbl = (const T *) fresh(ap)->lsum;
   int lds_off = 0;
-  if (S != SEMI_CYK) {
+  if (S != SEMI_CYK && TBL_LDS) {
     T *lt = (T *) lds_raw;
-    const T *gt = (const T *) a.lsum;
+    const T *gt = (const T *) fresh(ap)->lsum;
     for (int i = threadIdx.x; i < LOGSUM_N; i += blockDim.x) lt[i] = gt[i];
     __syncthreads();
     tbl = lt;
     lds_off = LOGSUM_N * 4;
   }
-  if (slot >= a.nslots) return;
+  if (slot >= fresh(ap)->nslots) return;
 
-  T   *acc_sc = (T *) (lds_raw + lds_off + (size_t) wib * a.acc_n * 8);
-  int *acc_r  = (int *) (acc_sc + a.acc_n);
+  const int acc_n = fresh(ap)->acc_n;
+  T   *acc_sc = (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);
+  int *acc_r  = (int *) (acc_sc + acc_n);
 
-  char *sbase = a.scratch + (size_t) slot * a.slot_bytes;
-  T   *deck = (T *) (sbase + a.deck_off);
-  T   *row0 = (T *) (sbase + a.row_off);
-  T   *row1 = row0 + a.R;
-  float *vscs = (float *) (sbase + a.vsc_off);
-  const T *esc   = (const T *) a.esc;
-  const T *rtmpl = (const T *) a.row_tmpl;
-  const T *dtmpl = (const T *) a.deck_tmpl;
-  const uint8_t *dsq = a.dsq;
-  const int64_t dsq_off = a.dsq_off;
-  const uint32_t el = a.el_selfsc;
-  const int rdn = a.root_dn;
+  char *sbase = fresh(ap)->scratch + (size_t) slot * fresh(ap)->slot_bytes;
+  T   *deck = (T *) (sbase + fresh(ap)->deck_off);
+  T   *row0 = (T *) (sbase + fresh(ap)->row_off);
+  float *vscs = (float *) (sbase + fresh(ap)->vsc_off);
 
-  if (VSC) for (int v = lane; v < a.M; v += 64) vscs[v] = IMPF;
+  if (VSC) for (int v = lane; v < fresh(ap)->M; v += 64) vscs[v] = IMPF;
 
-  for (int c = slot; c < a.nchunks; c += a.nslots) {
-    const int64_t ac = a.i0 + (int64_t) c * a.chunk_len;
-    const int64_t bc = min(a.j0, ac + a.chunk_len - 1);
-    const int64_t s0 = max(a.i0, ac - (int64_t) a.W + 1);
+  for (int c = slot; c < fresh(ap)->nchunks; c += fresh(ap)->nslots) {
+    const int64_t ac = fresh(ap)->i0 + (int64_t) c * fresh(ap)->chunk_len;
+    const int64_t bc = min(fresh(ap)->j0, ac + fresh(ap)->chunk_len - 1);
+    const int64_t s0 = max(fresh(ap)->i0, ac - (int64_t) fresh(ap)->W + 1);
+    const int R = fresh(ap)->R;
+    T *row1 = row0 + R;
 
-    for (int x = lane; x < a.R; x += 64) { T t = rtmpl[x]; row0[x] = t; row1[x] = t; }
-    for (int x = lane; x < a.RB; x += 64) deck[x] = dtmpl[x];
+    {
+      const T *rtmpl = (const T *) fresh(ap)->row_tmpl;
+      for (int x = lane; x < R; x += 64) { T t = rtmpl[x]; row0[x] = t; row1[x] = t; }
+      const T *dtmpl = (const T *) fresh(ap)->deck_tmpl;
+      const int RB = fresh(ap)->RB;
+      for (int x = lane; x < RB; x += 64) deck[x] = dtmpl[x];
+    }
     wave_sync();
 
     float   best = IMPF;
@@ -291,139 +450,70 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
       const int jp   = (int) (j - s0 + 1);
       T *cur = (j & 1) ? row1 : row0;
       T *prv = (j & 1) ? row0 : row1;
-      const int xj = __builtin_amdgcn_readfirstlane((int) dsq[j - dsq_off]);
-      const uint8_t *dsq_j = dsq + (j + 1 - dsq_off);   /* left residue of cell d: dsq_j[-d] */
-      const int rdx = min(jp, a.root_dx);
+      const uint8_t *dsq_j = fresh(ap)->dsq + (j + 1 - fresh(ap)->dsq_off);   /* left residue of cell d: dsq_j[-d] */
+      const int xj = __builtin_amdgcn_readfirstlane((int) dsq_j[-1]);
+      const int rdn = fresh(ap)->root_dn;
+      const int rdx = min(jp, fresh(ap)->root_dx);
+      const int local_begin = fresh(ap)->local_begin;
+      const uint32_t el = fresh(ap)->el_selfsc;
+      const DState *stp = fresh(ap)->st;
+      const T *esc = (const T *) fresh(ap)->esc;
+      const int nst = fresh(ap)->nst;
 
-      if (a.local_begin) {
+      if (local_begin) {
         for (int d = rdn + lane; d <= rdx; d += 64) { acc_sc[d - rdn] = R::neg_sentinel(); acc_r[d - rdn] = 0; }
       }
 
-      for (int v = a.M - 1; v >= 1; v--) {
-        const DState s = load_state(a.st + v);
-        const int kind = s.kind;
-        if (kind == KIND_SKIP) continue;
-        const int dn = s.dn;
-        const int dx = min(jp, s.dx);
-        if (dn > dx) continue;
-        const int has_begin = a.local_begin && s.has_begin;
-        const int bdn = max(rdn, dn), bdx = min(rdx, dx);
-        const T bsc = R::bits(s.beginsc);
-        float vmax = IMPF;
+      const int dsq_w = fresh(ap)->W + 64;
+      const uint8_t *dsq_b = dsq_j - dsq_w;   /* left residue of cell d: dsq_b[dsq_w - d] */
 
-        if (kind == KIND_GENERIC || kind == KIND_BEGL) {
-          const T *crow = s.rdelta ? prv : cur;
-          T *out = (kind == KIND_BEGL) ? deck + (jp & s.ring_mask) * s.ring_stride : cur;
-          const int sd = s.sd, em = s.emitmode;
-          /* emission row: lane y <- score of residue code y (MP: left code y, right code xj) */
-          int ev = 0;
-          if (em != EMITNONE) {
-            const int yy = lane < KP ? lane : 0;
-            ev = R::tob(esc[s.esc_off + (em == EMITPAIR ? yy * KP + xj : yy)]);
-          }
-          const T eright = (em == EMITRIGHT) ? R::fromb(__builtin_amdgcn_readlane(ev, xj)) : R::imp();
-          for (int d0 = dn; d0 <= dx; d0 += 64) {
-            const int d = d0 + lane;
-            const bool act = d <= dx;
-            const int dd = act ? d : dx;
-            const int dy = dd - sd;
-            const int xl = (em == EMITLEFT || em == EMITPAIR) ? (int) dsq_j[-dd] : 0;
-            const T ini = R::init(s.init_ok, el, s.endsc, dy);
-            T sc;
-            if (kind == KIND_BEGL) sc = block_dispatch<S, true, false>(s.cnum, crow, s.child_base, s.tsc, dy, ini, tbl);
-            else                   sc = block_dispatch<S, false, false>(s.cnum, crow, s.child_base, s.tsc, dy, ini, tbl);
-            if (em == EMITLEFT || em == EMITPAIR) sc = R::add(sc, pick_esc<S>(ev, xl));
-            else if (em == EMITRIGHT)             sc = R::add(sc, eright);
-            if (act) {
-              out[s.base + d] = sc;
-              if (has_begin && d >= bdn && d <= bdx) {
-                const T cand = R::add(sc, bsc);
-                if (cand >= acc_sc[d - rdn]) { acc_sc[d - rdn] = cand; acc_r[d - rdn] = v; }
-              }
-              if (VSC) vmax = fmaxf(vmax, R::score(sc));
-            }
+      for (int i = 0; i < nst; i++) {
+        const DState s = load_state(stp + i);
+        const uint32_t hdr = s.hdr;
+        const int kind = hdr & 7;
+        const int dn = s.dnx & 0xffff;
+        const int dx = min(jp, (int) (s.dnx >> 16));
+        if (dn > dx) continue;
+        const int v = hdr >> 16;
+        const bool has_begin = local_begin && ((hdr >> 11) & 1);
+        const int cnum = (hdr >> 8) & 7;
+        const int em = (hdr >> 3) & 3;
+        const int bdn = max(rdn, dn), bdx = min(rdx, dx);
+        /* emission row: lane y <- score of residue code y (MP: left code y, right code xj) */
+        int ev = 0;
+        if (em != EMITNONE) {
+          const int yy = lane < KP ? lane : 0;
+          ev = R::tob(esc[s.aux + (em == EMITPAIR ? yy * KP + xj : yy)]);
+        }
+        float vmax;
+        if (kind == KIND_GENERIC) {
+          switch (cnum) {
+          case 2:  vmax = gen_state<S, VSC, 2, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 3:  vmax = gen_state<S, VSC, 3, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 4:  vmax = gen_state<S, VSC, 4, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 5:  vmax = gen_state<S, VSC, 5, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          default: vmax = gen_state<S, VSC, 6, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
           }
         } else if (kind == KIND_IL) {
-          /* x_d = op(pre_d, x_{d-1} + t0) + e_d, pre_d = the other terms */
-          const T t0 = R::bits(s.tsc[0]);
-          const int yy = lane < KP ? lane : 0;
-          const int ev = R::tob(esc[s.esc_off + yy]);
-          T carry = cur[s.base + dn - 1];
-          for (int d0 = dn; d0 <= dx; d0 += 64) {
-            const int d = d0 + lane;
-            const bool act = d <= dx;
-            const int dd = act ? d : dx;            /* inactive lanes mirror a valid cell */
-            const int dy = dd - 1;
-            const int xl = (int) dsq_j[-dd];
-            const T ini = R::init(s.init_ok, el, s.endsc, dy);
-            const T pre = block_dispatch<S, false, true>(s.cnum, cur, s.child_base, s.tsc, dy, ini, tbl);
-            const T e   = pick_esc<S>(ev, xl);
-            const int cb = R::tob(carry);
-            T x = R::add(R::op(pre, R::add(lane == 0 ? carry : R::imp(), t0), tbl), e);
-            for (int it = 0; it < 65; it++) {
-              const T left = R::fromb(shr1(cb, R::tob(x)));
-              const T xn = R::add(R::op(pre, R::add(left, t0), tbl), e);
-              const bool changed = act && !R::same(xn, x);
-              x = xn;
-              if (!__any(changed)) break;
-            }
-            if (act) {
-              cur[s.base + d] = x;
-              if (has_begin && d >= bdn && d <= bdx) {
-                const T cand = R::add(x, bsc);
-                if (cand >= acc_sc[d - rdn]) { acc_sc[d - rdn] = cand; acc_r[d - rdn] = v; }
-              }
-              if (VSC) vmax = fmaxf(vmax, R::score(x));
-            }
-            carry = R::fromb(__builtin_amdgcn_readlane(R::tob(x), 63));
+          switch (cnum) {
+          case 2:  vmax = il_state<S, VSC, 2>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 3:  vmax = il_state<S, VSC, 3>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 4:  vmax = il_state<S, VSC, 4>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 5:  vmax = il_state<S, VSC, 5>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          default: vmax = il_state<S, VSC, 6>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
           }
-        } else { /* KIND_B */
-          const int by = s.child_base[1];
-          const int rmask = s.ring_mask, rstride = s.ring_stride;
-          const T *wbase = deck + s.child_base[0];
-          for (int d0 = dn; d0 <= dx; d0 += 64) {
-            const int d = d0 + lane;
-            const bool act = d <= dx;
-            const int dd = act ? d : dx;
-            const int dl = min(d0 + 63, dx);
-            int kmin, kmax, K0, K1;
-            if (s.banded) {
-              kmin = max(0, max(s.kdn_y, dd - s.kdx_w));
-              kmax = min(s.kdx_y, dd - s.kdn_w);
-              K0 = max(0, max(s.kdn_y, d0 - s.kdx_w));   /* both bounds grow with d */
-              K1 = min(s.kdx_y, dl - s.kdn_w);
-            } else { kmin = 0; kmax = dd; K0 = 0; K1 = dl; }
-            T sc = R::init(s.init_ok, el, s.endsc, dd);
-            for (int kb = K0; kb <= K1; kb += 64) {
-              const int ke = min(K1, kb + 63);
-              const int yb = R::tob(cur[by + min(kb + lane, ke)]);
-              /* four k per step, loads first; k > ke is masked (its reads stay
-               * inside the slot's guard cells) */
-              for (int k = kb; k <= ke; k += 4) {
-                T bl[4], yv[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                  const int kk = k + u;
-                  yv[u] = R::fromb(__builtin_amdgcn_readlane(yb, min(kk - kb, 63)));
-                  bl[u] = wbase[((jp - kk) & rmask) * rstride - kk + dd];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                  const int kk = k + u;
-                  const T nsc = R::op(sc, R::add(bl[u], yv[u]), tbl);
-                  sc = (kk <= ke && kk >= kmin && kk <= kmax) ? nsc : sc;
-                }
-              }
-            }
-            if (act) {
-              cur[s.base + d] = sc;
-              if (has_begin && d >= bdn && d <= bdx) {
-                const T cand = R::add(sc, bsc);
-                if (cand >= acc_sc[d - rdn]) { acc_sc[d - rdn] = cand; acc_r[d - rdn] = v; }
-              }
-              if (VSC) vmax = fmaxf(vmax, R::score(sc));
-            }
+        } else if (kind == KIND_BEGL) {
+          T *out = deck + (jp & (s.aux & 0xffff)) * (int) (s.aux >> 16);
+          switch (cnum) {
+          case 1:  vmax = gen_state<S, VSC, 1, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 2:  vmax = gen_state<S, VSC, 2, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 3:  vmax = gen_state<S, VSC, 3, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 4:  vmax = gen_state<S, VSC, 4, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          case 5:  vmax = gen_state<S, VSC, 5, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+          default: vmax = gen_state<S, VSC, 6, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
           }
+        } else {
+          vmax = b_state<S, VSC>(CTX_ARGS, s, cur, deck, dn, dx, has_begin, bdn, bdx, v);
         }
         if (VSC) {
           for (int o = 32; o >= 1; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
@@ -434,6 +524,10 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
 
       /* ROOT_S (src/cm_dpsearch.c:562-623) and hit candidates */
       if (j >= ac) {
+        CArgs *aq = fresh(ap);
+        const int root_cnum = aq->root_cnum;
+        const int cand_mode = aq->cand_mode;
+        const float cutoff = aq->cutoff, env_cutoff = aq->env_cutoff;
         float carry_max = -INFINITY;   /* prefix max of reportable scores (record filter) */
         for (int d0 = rdn; d0 <= rdx; d0 += 64) {
           const int d = d0 + lane;
@@ -441,20 +535,20 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
           float bs = IMPF;
           int   br = 0;
           if (act) {
-            T al = R::clamp_root(R::add(cur[a.root_child_base[0] + d], R::bits(a.root_tsc[0])));
-            for (int k = 1; k < a.root_cnum; k++)
-              al = R::op(al, R::add(cur[a.root_child_base[k] + d], R::bits(a.root_tsc[k])), tbl);
-            if (a.local_begin) {
+            T al = R::clamp_root(R::add(cur[(uint32_t) (aq->root_child_base[0] + d)], R::bits(aq->root_tsc[0])));
+            for (int k = 1; k < root_cnum; k++)
+              al = R::op(al, R::add(cur[(uint32_t) (aq->root_child_base[k] + d)], R::bits(aq->root_tsc[k])), tbl);
+            if (local_begin) {
               const T acv = acc_sc[d - rdn];
               if (acv > al) { al = acv; br = acc_r[d - rdn]; }
             }
             bs = R::score(al);
             best = fmaxf(best, bs);
-            if (R::envval(al) >= a.env_cutoff) { envi = min(envi, j - d + 1); envj = max(envj, j); }
+            if (R::envval(al) >= env_cutoff) { envi = min(envi, j - d + 1); envj = max(envj, j); }
           }
           bool rep;
-          const bool q = act && bs >= a.cutoff && bs > NOTIMP_F;
-          if (a.cand_mode == 1) {
+          const bool q = act && bs >= cutoff && bs > NOTIMP_F;
+          if (cand_mode == 1) {
             float xq = q ? bs : -INFINITY;
             float inc = xq;
             for (int o = 1; o < 64; o <<= 1) {
@@ -466,7 +560,7 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
             const float pmax = fmaxf(IMPF, fmaxf(carry_max, ex));
             rep = q && bs > pmax;
             carry_max = fmaxf(carry_max, __shfl(inc, 63, 64));
-          } else if (a.cand_mode == 2) {
+          } else if (cand_mode == 2) {
             rep = act && bs > 0.f && bs > NOTIMP_F;
           } else {
             rep = q;
@@ -475,14 +569,14 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
           if (m) {
             const int n = __popcll(m);
             unsigned long long basei = 0;
-            if (lane == 0) basei = atomicAdd(a.ncand, (unsigned long long) n);
+            if (lane == 0) basei = atomicAdd(aq->ncand, (unsigned long long) n);
             basei = __shfl(basei, 0, 64);
             if (rep) {
               const unsigned long long below = m & ((1ull << lane) - 1ull);
               const unsigned long long idx = basei + __popcll(below);
-              if ((int64_t) idx < a.cand_cap) {
+              if ((int64_t) idx < aq->cand_cap) {
                 Cand cd; cd.j = (int32_t) j; cd.d = d; cd.sc = bs; cd.r = br;
-                a.cand[idx] = cd;
+                aq->cand[idx] = cd;
               }
             }
           }
@@ -496,59 +590,70 @@ __global__ void __launch_bounds__(64 * WPB_MAX) std_scan_kernel(ScanArgs a)
       envi = min(envi, (int64_t) __shfl_xor((long long) envi, o, 64));
       envj = max(envj, (int64_t) __shfl_xor((long long) envj, o, 64));
     }
-    if (lane == 0) { a.chunk_best[c] = best; a.chunk_envi[c] = envi; a.chunk_envj[c] = envj; }
+    if (lane == 0) { CArgs *aq = fresh(ap); aq->chunk_best[c] = best; aq->chunk_envi[c] = envi; aq->chunk_envj[c] = envj; }
   } /* chunks */
 }
+
+/* Kernels.  CYK and Inside with the log-sum table read from L1/L2 are
+ * register-allocated for 8 wavefronts per SIMD; Inside with the table in LDS
+ * is limited to one 16-wavefront workgroup per CU by the table itself. */
+template <bool VSC>
+__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_cyk_kernel(const ScanArgs *a)
+{ std_scan_body<SEMI_CYK, VSC, false>(a); }
+
+template <int S, bool VSC>
+__global__ void __launch_bounds__(64 * WPB_MAX) std_inside_lds_kernel(const ScanArgs *a)
+{ std_scan_body<S, VSC, true>(a); }
+
+template <int S, bool VSC>
+__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_inside_l2_kernel(const ScanArgs *a)
+{ std_scan_body<S, VSC, false>(a); }
 
 /* ------------------------------------------------------------------------ */
 /* launch stubs for scan_host.cpp                                            */
 
-static const void *std_kernel_ptr(int semi, int want_vsc)
+static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds)
 {
-  switch (semi * 2 + (want_vsc ? 1 : 0)) {
-  case 0: return (const void *) std_scan_kernel<SEMI_CYK, false>;
-  case 1: return (const void *) std_scan_kernel<SEMI_CYK, true>;
-  case 2: return (const void *) std_scan_kernel<SEMI_FINSIDE, false>;
-  case 3: return (const void *) std_scan_kernel<SEMI_FINSIDE, true>;
-  case 4: return (const void *) std_scan_kernel<SEMI_IINSIDE, false>;
-  case 5: return (const void *) std_scan_kernel<SEMI_IINSIDE, true>;
-  default: return NULL;
+  if (semi == SEMI_CYK) return want_vsc ? (const void *) std_cyk_kernel<true> : (const void *) std_cyk_kernel<false>;
+  if (semi == SEMI_FINSIDE) {
+    if (tbl_lds) return want_vsc ? (const void *) std_inside_lds_kernel<SEMI_FINSIDE, true> : (const void *) std_inside_lds_kernel<SEMI_FINSIDE, false>;
+    return want_vsc ? (const void *) std_inside_l2_kernel<SEMI_FINSIDE, true> : (const void *) std_inside_l2_kernel<SEMI_FINSIDE, false>;
   }
+  if (semi == SEMI_IINSIDE) {
+    if (tbl_lds) return want_vsc ? (const void *) std_inside_lds_kernel<SEMI_IINSIDE, true> : (const void *) std_inside_lds_kernel<SEMI_IINSIDE, false>;
+    return want_vsc ? (const void *) std_inside_l2_kernel<SEMI_IINSIDE, true> : (const void *) std_inside_l2_kernel<SEMI_IINSIDE, false>;
+  }
+  return NULL;
 }
 
 /* LDS bytes of a workgroup of `wpb` wavefronts */
-extern "C" size_t infx_std_lds_bytes(int semi, int wpb, int acc_n)
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n)
 {
-  return (semi != SEMI_CYK ? (size_t) LOGSUM_N * 4 : 0) + (size_t) wpb * acc_n * 8;
+  return (semi != SEMI_CYK && tbl_lds ? (size_t) LOGSUM_N * 4 : 0) + (size_t) wpb * acc_n * 8;
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
-extern "C" int infx_std_occupancy(int semi, int want_vsc, int wpb, int acc_n)
+extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n)
 {
-  const void *f = std_kernel_ptr(semi, want_vsc);
+  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds);
   if (!f || wpb < 1 || wpb > WPB_MAX) return 0;
-  const size_t lds = infx_std_lds_bytes(semi, wpb, acc_n);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, wpb, acc_n);
   if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   return nb;
 }
 
-extern "C" hipError_t infx_launch_std(int semi, int want_vsc, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a)
+extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, dim3 grid, dim3 block, hipStream_t stream,
+                                      const ScanArgs *a, int acc_n)
 {
-  const void *f = std_kernel_ptr(semi, want_vsc);
+  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds);
   if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
-  const size_t lds = infx_std_lds_bytes(semi, block.x / 64, a->acc_n);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n);
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
-  switch (semi * 2 + (want_vsc ? 1 : 0)) {
-  case 0: hipLaunchKernelGGL((std_scan_kernel<SEMI_CYK, false>), grid, block, lds, stream, *a); break;
-  case 1: hipLaunchKernelGGL((std_scan_kernel<SEMI_CYK, true>), grid, block, lds, stream, *a); break;
-  case 2: hipLaunchKernelGGL((std_scan_kernel<SEMI_FINSIDE, false>), grid, block, lds, stream, *a); break;
-  case 3: hipLaunchKernelGGL((std_scan_kernel<SEMI_FINSIDE, true>), grid, block, lds, stream, *a); break;
-  case 4: hipLaunchKernelGGL((std_scan_kernel<SEMI_IINSIDE, false>), grid, block, lds, stream, *a); break;
-  case 5: hipLaunchKernelGGL((std_scan_kernel<SEMI_IINSIDE, true>), grid, block, lds, stream, *a); break;
-  default: return hipErrorInvalidValue;
-  }
+  void *args[] = { (void *) &a };
+  e = hipLaunchKernel(f, grid, block, args, lds, stream);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }

@@ -242,13 +242,14 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
     maxlen = ESL_MAX(maxlen, wj0[w] - wi0[w] + 1);
     total += wj0[w] - wi0[w] + 1;
   }
+  /* profile (host only: its errors are reported with or without a device) */
+  TrProfile fresh;
+  if ((status = build_tr_profile(cm, trsmx, qdbidx, semi, pty, fresh, errbuf)) != eslOK) return status;
+
   if ((status = infx_dev_init(errbuf)) != eslOK) return status;
   hipStream_t stream = infx_stream();
   if (!trsmx->dev) trsmx->dev = new TrDev();
   TrDev *D = (TrDev *) trsmx->dev;
-
-  TrProfile fresh;
-  if ((status = build_tr_profile(cm, trsmx, qdbidx, semi, pty, fresh, errbuf)) != eslOK) return status;
   TrProfile *P = NULL;
   for (auto *p : D->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->pty == pty) P = p;
   if (!P) { P = new TrProfile(); D->prof.push_back(P); }

@@ -195,3 +195,31 @@ def test_cells_per_residue_match_shipped_model_table():
         cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=True)
         c1, o1 = cm.cells_per_res(1)
         assert (c1, o1) == (cells, ops), (name, c1, o1)
+
+
+@pytest.mark.skipif(ia.gpu_available(), reason="without a GPU the scanners stop right after building their device profile")
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Vault.c.cm", "5S_rRNA.c.cm", "Cobalamin.c.cm", "Plant_SRP.c.cm",
+                                  "rnaseP-eubact.c.cm", "tRNA5.c.cm"])
+def test_scan_profiles_build_for_every_model_and_length(name):
+    """The host-side scan profiles (packed state descriptors, deck rings,
+    truncated penalty lists) build for every shipped model, both modes, all
+    band sets and target lengths around W: the only error left is the
+    missing device."""
+    rng = np.random.default_rng(0)
+    for local in (True, False):
+        cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=local, trunc=True, trscanmx=True)
+        W = cm.W
+        for L in (1, 2, 7, W - 1, W, W + 1, 3 * W):
+            if L < 1:
+                continue
+            dsq = ia.dsq_from_codes(rng.integers(0, 4, L).astype(np.uint8))
+            for fn in (ia.FastCYKScan, ia.FastFInsideScan, ia.FastIInsideScan):
+                for q in (0, 1, 2):
+                    with pytest.raises(ia.ScanError) as e:
+                        fn(cm, q, dsq, 1, L, 0.0)
+                    assert e.value.status == ia.eslENODEVICE, (name, local, L, fn.__name__, q, str(e.value))
+            for fn in (ia.RefTrCYKScan, ia.RefITrInsideScan):
+                for p in (ia.PLI_PASS_5P_ONLY_FORCE, ia.PLI_PASS_5P_AND_3P_ANY):
+                    with pytest.raises(ia.ScanError) as e:
+                        fn(cm, 1, p, dsq, 1, L, 0.0)
+                    assert e.value.status == ia.eslENODEVICE, (name, local, L, fn.__name__, p, str(e.value))
