@@ -377,6 +377,15 @@ int  infx_gpu_available(char *errbuf);                 /* 1 if a gfx9 HIP device
  * call, and the summed device duration of the DP kernel launches in it
  * (HIP events on the library's stream). */
 void infx_last_call_timing(double *ret_wall_ms, double *ret_kernel_ms, int64_t *ret_cells, int *ret_launches);
+/* The device scan profile the standard scanners would build for a target of
+ * length L: up to `cap` 64-byte state descriptors (every state but E and
+ * ROOT_S, v descending), the row and deck sizes in cells, the emission table
+ * size, and the root band and children (ret_root[0..2] = dn, dx, cnum;
+ * [3..8] = child storage bases).  Returns the descriptor count, or -status.
+ * No counterpart in the reference: used to check the kernels' address
+ * arithmetic on the host. */
+int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int64_t L, void *out_st, int cap,
+                           int *ret_R, int *ret_RB, int *ret_nesc, int *ret_root, char *errbuf);
 /* Override the work split (0 = automatic): residues per chunk and waves per
  * block.  Used by the autotuner and tests. */
 void infx_set_launch_params(int chunk_len, int waves_per_block);

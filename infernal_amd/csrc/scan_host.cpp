@@ -395,7 +395,8 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
       return infx_fail(eslERANGE, errbuf, "state %d does not fit the packed scan descriptor (M, cnum or band too large)", v);
     s.hdr = (uint32_t) kind | ((uint32_t) Emitmode(st) << 3) | ((uint32_t) StateDelta(st) << 5) |
             ((uint32_t) (StateRightDelta(st) > 0) << 7) | ((uint32_t) cnum << 8) | ((uint32_t) has_begin << 11) |
-            ((uint32_t) init_ok << 12) | ((uint32_t) banded << 13) | ((uint32_t) v << 16);
+            ((uint32_t) init_ok << 12) | ((uint32_t) banded << 13) |
+            ((uint32_t) (StateRightDelta(st) > 0 && cm->cfirst[v] == v) << 14) | ((uint32_t) v << 16);
     s.dnx = (uint32_t) dn[v] | ((uint32_t) dx[v] << 16);
     s.base = base[v];
     if (st == ML_st || st == MR_st || st == IL_st || st == IR_st || st == MP_st) {
@@ -406,6 +407,7 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
     if (semi == SEMI_IINSIDE) { s.beginsc = (uint32_t) cm->ibeginsc[v]; s.endsc = (uint32_t) cm->iendsc[v]; }
     else                      { s.beginsc = fbits(cm->beginsc[v]);       s.endsc = fbits(cm->endsc[v]); }
     if (cm->stid[v] == BEGL_S) {
+      if (has_begin) return infx_fail(eslEINVAL, errbuf, "BEGL_S state %d has a local begin score", v);
       if (!fits16(ring[v] - 1) || !fits16(stride[v])) return infx_fail(eslERANGE, errbuf, "BEGL deck of state %d too large", v);
       s.aux = (uint32_t) (ring[v] - 1) | ((uint32_t) stride[v] << 16);
     }
@@ -558,11 +560,11 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   const int64_t nchunks64 = (L + chunk - 1) / chunk;
   if (nchunks64 > INT32_MAX / 2) return infx_fail(eslERANGE, errbuf, "%s: too many chunks", fname);
   const int nchunks = (int) nchunks64;
-  /* slot: guard | deck | guard | row0 | row1 | guard | vsc  (4-byte cells) */
+  /* slot: guard | deck | guard | rows[4] (ring over j) | guard | vsc  (4-byte cells) */
   const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
   const size_t deck_off = G * 4;
   const size_t row_off  = deck_off + ((size_t) P->RB + G) * 4;
-  const size_t vsc_off  = row_off + ((size_t) 2 * P->R + G) * 4;
+  const size_t vsc_off  = row_off + ((size_t) 4 * P->R + G) * 4;
   const size_t slot_bytes = ((vsc_off + (size_t) cm->M * 4) + 255) / 256 * 256;
   size_t mem_free = 0, mem_total = 0;
   if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
@@ -572,7 +574,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
 
-  if ((status = D->d_dsq.reserve((size_t) L + 64)) != eslOK ||
+  if ((status = D->d_dsq.reserve((size_t) L + 128)) != eslOK ||
       (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
       (status = D->d_chunk.reserve((size_t) nchunks * 24 + 64)) != eslOK ||
       (status = D->d_ncand.reserve(64)) != eslOK)
@@ -580,7 +582,10 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
   if ((status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
-  if (hipMemcpyAsync(D->d_dsq.ptr, dsq + i0, (size_t) L, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+  /* target residues i0..j0 behind and before 64 zero bytes (residue codes
+   * past j0 are read, and ignored, by idle lanes of the last row pair) */
+  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) L + 128, g_stream) != hipSuccess ||
+      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) L, hipMemcpyHostToDevice, g_stream) != hipSuccess)
     return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
 
   ScanArgs A;
@@ -588,7 +593,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.st = (const DState *) P->d_st.ptr;
   A.nst = (int) P->st.size();
   A.esc = P->d_esc.ptr;
-  A.dsq = (const uint8_t *) D->d_dsq.ptr;
+  A.dsq = (const uint8_t *) D->d_dsq.ptr + 64;
   A.dsq_off = i0;
   A.row_tmpl = P->d_rtmpl.ptr;
   A.deck_tmpl = P->d_dtmpl.ptr;
@@ -699,6 +704,26 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   infx_timing_add(wall, kernel_ms_total, (int64_t) cells, launches);
   return eslOK;
+}
+
+/* The device scan profile FastCYKScan/FastFInsideScan/FastIInsideScan would
+ * upload for a target of length L (for layout checks on the host):
+ * descriptors (up to cap), row/deck sizes, emission-table size, root band
+ * and children.  Returns the number of descriptors, or -status on error. */
+extern "C" int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int64_t L, void *out_st, int cap,
+                                      int *ret_R, int *ret_RB, int *ret_nesc, int *ret_root /* [3 + 6]: dn, dx, cnum, child bases */,
+                                      char *errbuf)
+{
+  if (!smx || L < 1) return -eslEINVAL;
+  const int W = (smx->W > L) ? (int) L : smx->W;
+  StdProfile P;
+  int status = build_std_profile(cm, smx, qdbidx, semi, W, P, errbuf);
+  if (status != eslOK) return -status;
+  for (int i = 0; i < (int) P.st.size() && i < cap; i++) memcpy((char *) out_st + (size_t) i * sizeof(DState), &P.st[i], sizeof(DState));
+  *ret_R = P.R; *ret_RB = P.RB; *ret_nesc = (int) P.esc.size();
+  ret_root[0] = P.root_dn; ret_root[1] = P.root_dx; ret_root[2] = P.root_cnum;
+  for (int k = 0; k < 6; k++) ret_root[3 + k] = P.root_child_base[k];
+  return (int) P.st.size();
 }
 
 /* Exact number of (v, j, d) cells the reference computes over j = i0..j0

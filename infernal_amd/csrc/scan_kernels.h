@@ -38,7 +38,8 @@ enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };
 /* 64 bytes: one scalar load.  hdr bits: kind 0-2, emitmode 3-4, sd 5-6,
  * rdelta 7 (children read from the j-1 row), cnum 8-10, has_begin 11,
  * init_ok 12 (endsc finite: init(d) = el*d + endsc), banded 13 (B: QDB k
- * bounds, else 0..d), state index v 16-31. */
+ * bounds, else 0..d), self_prev 14 (the state is its own child on the j-1
+ * row: IR), state index v 16-31. */
 struct DState {
   uint32_t hdr;
   uint32_t dnx;        /* dn | dx << 16: band at steady state; per-j dx = min(jp, dx) */

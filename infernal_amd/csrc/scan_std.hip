@@ -209,68 +209,98 @@ __device__ __forceinline__ typename SR<S>::T pick_esc(int ev_bits, int x)
   return SR<S>::fromb(__builtin_amdgcn_ds_bpermute(x << 2, ev_bits));
 }
 
-/* Per-j context shared by the state loops below, passed as plain values
- * (a struct here would be selected by field address and spilled). */
-#define CTX_PARAMS const typename SR<S>::T *cur_, const typename SR<S>::T *prv_, typename SR<S>::T *acc_sc, int *acc_r, \
-                   const typename SR<S>::T *tbl, const uint8_t *dsq_b, int dsq_w, int xj, int jp, int rdn, uint32_t el, int lane
-#define CTX_ARGS   cur, prv, acc_sc, acc_r, tbl, dsq_b, dsq_w, xj, jp, rdn, el, lane
+/* ------------------------------------------------------------------------ */
+/* Two-row sweeps.  A wavefront computes rows j and j+1 of a state before it
+ * moves to the next state: every cell of row j+1 depends only on higher
+ * states (rows j, j+1), the same state's row j+1 (the IL self-loop) and rows
+ * <= j, so the order is the reference's order for each row, and the state
+ * descriptor, emission row and loop overheads are paid once per two rows.
+ * The row storage is a ring of 4 rows (j & 3): rows j-1, j and j+1 live. */
 
-/* local-begin candidate of cell d (src/cm_dpsearch.c:549-560): the last
- * state with the largest alpha + beginsc wins ties (v descends) */
+/* Per-sweep context passed as plain values (a struct would be selected by
+ * field address and spilled to scratch). */
+#define SW_PARAMS const typename SR<S>::T *tbl, const uint8_t *dsq_b0, int dsq_w, int xj0, int xj1, uint32_t el, int lane, \
+                  typename SR<S>::T *acc0, typename SR<S>::T *acc1, int rdn
+#define SW_ARGS   tbl, dsq_b0, dsq_w, xj0, xj1, el, lane, acc0, acc1, rdn
+
+/* local-begin candidate of cell d (src/cm_dpsearch.c:599-612): the largest
+ * alpha + beginsc over begin states; the state is found again only for
+ * reported cells (begin_argmax) */
 template <int S>
-__device__ __forceinline__ void begin_update(typename SR<S>::T *acc_sc, int *acc_r, int rdn, bool on, int d,
-                                             typename SR<S>::T sc, typename SR<S>::T bsc, int v)
+__device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bool on, int d, typename SR<S>::T sc, typename SR<S>::T bsc)
 {
   typedef SR<S> R;
   if (on) {
     const typename R::T cand = R::add(sc, bsc);
-    if (cand >= acc_sc[d - rdn]) { acc_sc[d - rdn] = cand; acc_r[d - rdn] = v; }
+    if (cand >= acc[d - rdn]) acc[d - rdn] = cand;
   }
 }
 
-/* A non-B, non-IL state with CN children over its band [dn, dx] (64 cells
- * per block).  BEGL decks combine init, c0, c1, ... in child order
- * (src/cm_dpsearch.c:430-444); other states use combine_children. */
+/* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
+ * [dn, dx0] and [dn, dx1] (dx1 < dn: no row 1).  BEGL decks combine init, c0,
+ * c1, ... in child order (src/cm_dpsearch.c:430-444); other states use
+ * combine_children.  Returns the max score over the rows flagged in vs. */
 template <int S, bool VSC, int CN, bool BEGL>
-__device__ __forceinline__ float gen_state(CTX_PARAMS, const DState &s, typename SR<S>::T *out, int dn, int dx,
-                                           bool has_begin, int bdn, int bdx, int v, int ev)
+__device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
+                                            const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
+                                            typename SR<S>::T *out0, typename SR<S>::T *out1,
+                                            int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
+                                            int ev0, int ev1, bool vs0, bool vs1)
 {
   typedef SR<S> R;
   typedef typename R::T T;
   const uint32_t hdr = s.hdr;
   const int em = (hdr >> 3) & 3, sd = (hdr >> 5) & 3;
-  const T *crow = ((hdr >> 7) & 1) ? prv_ : cur_;
   const int init_ok = (hdr >> 12) & 1;
   const T bsc = R::bits(s.beginsc);
+  const int bdn = max(rdn, dn);
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
-  const T eright = R::fromb(__builtin_amdgcn_readlane(ev, xj));
+  const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
+  const T er1 = R::fromb(__builtin_amdgcn_readlane(ev1, xj1));
+  const uint8_t *dsq_b1 = dsq_b0 + 1;
+  const int dxm = max(dx0, dx1);
   float vmax = IMPF;
-  for (int d0 = dn; d0 <= dx; d0 += 64) {
+  for (int d0 = dn; d0 <= dxm; d0 += 64) {
     const int d = d0 + lane;
-    const bool act = d <= dx;
-    const int dd = act ? d : dx;
-    const int dy = dd - sd;
-    const int xl = (int) dsq_b[(uint32_t) (dsq_w - dd)];
-    T ch[CN];
+    const bool a0 = d <= dx0, a1 = d <= dx1;
+    const int dy0 = (a0 ? d : dn) - sd, dy1 = (a1 ? d : dn) - sd;   /* idle lanes read a valid cell */
+    /* left residue j-d+1 (idle lanes: residue j, inside the target even where
+     * the row's band is still empty at the start of a chunk) */
+    const int xl0 = (int) dsq_b0[(uint32_t) (dsq_w - (a0 ? d : 1))];
+    const int xl1 = (int) dsq_b1[(uint32_t) (dsq_w - (a1 ? d : 1))];
+    T c0[CN], c1[CN];
 #pragma unroll
-    for (int k = 0; k < CN; k++) ch[k] = R::add(crow[(uint32_t) (cb[k] + dy)], R::bits(s.tsc[k]));
-    const T ini = R::init(init_ok, el, s.endsc, dy);
-    T sc;
-    if (BEGL) {
-      sc = ini;
-#pragma unroll
-      for (int k = 0; k < CN; k++) sc = R::op(sc, ch[k], tbl);
-    } else {
-      sc = combine_children<S, CN>(ch, ini, true, tbl);
-      const T e = (em == EMITRIGHT) ? eright : pick_esc<S>(ev, xl);
-      if (em != EMITNONE) sc = R::add(sc, e);
+    for (int k = 0; k < CN; k++) {
+      c0[k] = crow0[(uint32_t) (cb[k] + dy0)];
+      c1[k] = crow1[(uint32_t) (cb[k] + dy1)];
     }
-    if (act) {
-      out[(uint32_t) (s.base + d)] = sc;
-      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, sc, bsc, v);
-      if (VSC) vmax = fmaxf(vmax, R::score(sc));
+#pragma unroll
+    for (int k = 0; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
+    const T ini0 = R::init(init_ok, el, s.endsc, dy0);
+    const T ini1 = R::init(init_ok, el, s.endsc, dy1);
+    T sc0, sc1;
+    if (BEGL) {
+      sc0 = ini0; sc1 = ini1;
+#pragma unroll
+      for (int k = 0; k < CN; k++) { sc0 = R::op(sc0, c0[k], tbl); sc1 = R::op(sc1, c1[k], tbl); }
+    } else {
+      sc0 = combine_children<S, CN>(c0, ini0, true, tbl);
+      sc1 = combine_children<S, CN>(c1, ini1, true, tbl);
+      const T e0 = (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0);
+      const T e1 = (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1);
+      if (em != EMITNONE) { sc0 = R::add(sc0, e0); sc1 = R::add(sc1, e1); }
+    }
+    if (a0) {
+      out0[(uint32_t) (s.base + d)] = sc0;
+      begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, sc0, bsc);
+      if (VSC && vs0) vmax = fmaxf(vmax, R::score(sc0));
+    }
+    if (a1) {
+      out1[(uint32_t) (s.base + d)] = sc1;
+      begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, sc1, bsc);
+      if (VSC && vs1) vmax = fmaxf(vmax, R::score(sc1));
     }
   }
   return vmax;
@@ -278,60 +308,81 @@ __device__ __forceinline__ float gen_state(CTX_PARAMS, const DState &s, typename
 
 /* IL state: x_d = op(pre_d, x_{d-1} + t0) + e_d with pre_d the other CN-1
  * child terms combined with init (the self term is applied last), solved
- * per 64-cell block by the exact fixed-point sweep. */
+ * per 64-cell block by the exact fixed-point sweep, both rows together. */
 template <int S, bool VSC, int CN>
-__device__ __forceinline__ float il_state(CTX_PARAMS, const DState &s, typename SR<S>::T *cur, int dn, int dx,
-                                          bool has_begin, int bdn, int bdx, int v, int ev)
+__device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename SR<S>::T *cur0, typename SR<S>::T *cur1,
+                                           int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
+                                           int ev, bool vs0, bool vs1)
 {
   typedef SR<S> R;
   typedef typename R::T T;
   const int init_ok = (s.hdr >> 12) & 1;
   const T bsc = R::bits(s.beginsc);
   const T t0 = R::bits(s.tsc[0]);
+  const int bdn = max(rdn, dn);
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
-  T carry = cur[(uint32_t) (s.base + dn - 1)];
+  const uint8_t *dsq_b1 = dsq_b0 + 1;
+  T carry0 = cur0[(uint32_t) (s.base + dn - 1)];
+  T carry1 = cur1[(uint32_t) (s.base + dn - 1)];
+  const int dxm = max(dx0, dx1);
   float vmax = IMPF;
-  for (int d0 = dn; d0 <= dx; d0 += 64) {
+  for (int d0 = dn; d0 <= dxm; d0 += 64) {
     const int d = d0 + lane;
-    const bool act = d <= dx;
-    const int dd = act ? d : dx;            /* inactive lanes mirror a valid cell */
-    const int dy = dd - 1;
-    const int xl = (int) dsq_b[(uint32_t) (dsq_w - dd)];
-    T ch[CN];
-    ch[0] = R::imp();
+    const bool a0 = d <= dx0, a1 = d <= dx1;
+    const int dd0 = a0 ? d : max(dn, dx0), dd1 = a1 ? d : max(dn, dx1);   /* idle lanes mirror a valid cell */
+    const int xl0 = (int) dsq_b0[(uint32_t) (dsq_w - (a0 ? d : 1))];   /* idle lanes: residue j */
+    const int xl1 = (int) dsq_b1[(uint32_t) (dsq_w - (a1 ? d : 1))];
+    T c0[CN], c1[CN];
+    c0[0] = R::imp(); c1[0] = R::imp();
 #pragma unroll
-    for (int k = 1; k < CN; k++) ch[k] = R::add(cur[(uint32_t) (cb[k] + dy)], R::bits(s.tsc[k]));
-    const T ini = R::init(init_ok, el, s.endsc, dy);
-    const T pre = combine_children<S, CN>(ch, ini, false, tbl);
-    const T e   = pick_esc<S>(ev, xl);
-    const int cbits = R::tob(carry);
-    T x = R::add(R::op(pre, R::add(lane == 0 ? carry : R::imp(), t0), tbl), e);
+    for (int k = 1; k < CN; k++) {
+      c0[k] = cur0[(uint32_t) (cb[k] + dd0 - 1)];
+      c1[k] = cur1[(uint32_t) (cb[k] + dd1 - 1)];
+    }
+#pragma unroll
+    for (int k = 1; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
+    const T pre0 = combine_children<S, CN>(c0, R::init(init_ok, el, s.endsc, dd0 - 1), false, tbl);
+    const T pre1 = combine_children<S, CN>(c1, R::init(init_ok, el, s.endsc, dd1 - 1), false, tbl);
+    const T e0 = pick_esc<S>(ev, xl0);
+    const T e1 = pick_esc<S>(ev, xl1);
+    const int cb0 = R::tob(carry0), cb1 = R::tob(carry1);
+    T x0 = R::add(R::op(pre0, R::add(lane == 0 ? carry0 : R::imp(), t0), tbl), e0);
+    T x1 = R::add(R::op(pre1, R::add(lane == 0 ? carry1 : R::imp(), t0), tbl), e1);
     for (int it = 0; it < 65; it++) {
-      const T left = R::fromb(shr1(cbits, R::tob(x)));
-      const T xn = R::add(R::op(pre, R::add(left, t0), tbl), e);
-      const bool changed = act && !R::same(xn, x);
-      x = xn;
+      const T l0 = R::fromb(shr1(cb0, R::tob(x0)));
+      const T l1 = R::fromb(shr1(cb1, R::tob(x1)));
+      const T n0 = R::add(R::op(pre0, R::add(l0, t0), tbl), e0);
+      const T n1 = R::add(R::op(pre1, R::add(l1, t0), tbl), e1);
+      const bool changed = (a0 && !R::same(n0, x0)) || (a1 && !R::same(n1, x1));
+      x0 = n0; x1 = n1;
       if (!__any(changed)) break;
     }
-    if (act) {
-      cur[(uint32_t) (s.base + d)] = x;
-      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, x, bsc, v);
-      if (VSC) vmax = fmaxf(vmax, R::score(x));
+    if (a0) {
+      cur0[(uint32_t) (s.base + d)] = x0;
+      begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, x0, bsc);
+      if (VSC && vs0) vmax = fmaxf(vmax, R::score(x0));
     }
-    carry = R::fromb(__builtin_amdgcn_readlane(R::tob(x), 63));
+    if (a1) {
+      cur1[(uint32_t) (s.base + d)] = x1;
+      begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, x1, bsc);
+      if (VSC && vs1) vmax = fmaxf(vmax, R::score(x1));
+    }
+    carry0 = R::fromb(__builtin_amdgcn_readlane(R::tob(x0), 63));
+    carry1 = R::fromb(__builtin_amdgcn_readlane(R::tob(x1), 63));
   }
   return vmax;
 }
 
-/* B state (src/cm_dpsearch.c:246-284): all lanes walk the union of their
- * k ranges together, so one k reads one BEGL deck row (consecutive columns)
- * and one BEGR cell (v_readlane broadcast); each lane keeps its own
+/* B state, one row (src/cm_dpsearch.c:246-284): all lanes walk the union of
+ * their k ranges together, so one k reads one BEGL deck row (consecutive
+ * columns) and one BEGR cell (v_readlane broadcast); each lane keeps its own
  * kmin(d)..kmax(d) in ascending order. */
 template <int S, bool VSC>
-__device__ __forceinline__ float b_state(CTX_PARAMS, const DState &s, typename SR<S>::T *cur, typename SR<S>::T *deck,
-                                         int dn, int dx, bool has_begin, int bdn, int bdx, int v)
+__device__ __forceinline__ float b_state(const typename SR<S>::T *tbl, uint32_t el, int lane, typename SR<S>::T *acc, int rdn,
+                                         const DState &s, typename SR<S>::T *cur, const typename SR<S>::T *deck,
+                                         int jp, int dn, int dx, bool hb, int bdx, bool vs)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -339,6 +390,7 @@ __device__ __forceinline__ float b_state(CTX_PARAMS, const DState &s, typename S
   const int init_ok = (hdr >> 12) & 1;
   const int banded = (hdr >> 13) & 1;
   const T bsc = R::bits(s.beginsc);
+  const int bdn = max(rdn, dn);
   const int kdn_y = s.tsc[0] & 0xffff, kdx_y = s.tsc[0] >> 16;
   const int kdn_w = s.tsc[1] & 0xffff, kdx_w = s.tsc[1] >> 16;
   const int by = (int) s.tsc[3];
@@ -382,14 +434,32 @@ __device__ __forceinline__ float b_state(CTX_PARAMS, const DState &s, typename S
     }
     if (act) {
       cur[(uint32_t) (s.base + d)] = sc;
-      begin_update<S>(acc_sc, acc_r, rdn, has_begin && d >= bdn && d <= bdx, d, sc, bsc, v);
-      if (VSC) vmax = fmaxf(vmax, R::score(sc));
+      begin_update<S>(acc, rdn, hb && d >= bdn && d <= bdx, d, sc, bsc);
+      if (VSC && vs) vmax = fmaxf(vmax, R::score(sc));
     }
   }
   return vmax;
 }
 
-/* ------------------------------------------------------------------------ */
+/* The local-begin state of reported cells whose root score came from a
+ * local begin: the same scan as the accumulation (states in descending v,
+ * ties to the later state), on the row that is still in the ring. */
+template <int S>
+__device__ __forceinline__ int begin_argmax(const DState *stp, int nst, const typename SR<S>::T *cur, int jp, int d)
+{
+  typedef SR<S> R;
+  typename R::T bestv = R::neg_sentinel();
+  int br = 0;
+  for (int i = 0; i < nst; i++) {
+    const DState s = load_state(stp + i);
+    if (!((s.hdr >> 11) & 1)) continue;
+    const int dn = s.dnx & 0xffff, dx = min(jp, (int) (s.dnx >> 16));
+    if (d < dn || d > dx) continue;
+    const typename R::T cand = R::add(cur[(uint32_t) (s.base + d)], R::bits(s.beginsc));
+    if (cand >= bestv) { bestv = cand; br = (int) (s.hdr >> 16); }
+  }
+  return br;
+}
 
 template <int S, bool VSC, bool TBL_LDS>
 __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
@@ -417,12 +487,12 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   if (slot >= fresh(ap)->nslots) return;
 
   const int acc_n = fresh(ap)->acc_n;
-  T   *acc_sc = (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);
-  int *acc_r  = (int *) (acc_sc + acc_n);
+  T *acc0 = (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);   /* local-begin maxima of rows j, j+1 */
+  T *acc1 = acc0 + acc_n;
 
   char *sbase = fresh(ap)->scratch + (size_t) slot * fresh(ap)->slot_bytes;
   T   *deck = (T *) (sbase + fresh(ap)->deck_off);
-  T   *row0 = (T *) (sbase + fresh(ap)->row_off);
+  T   *rows = (T *) (sbase + fresh(ap)->row_off);
   float *vscs = (float *) (sbase + fresh(ap)->vsc_off);
 
   if (VSC) for (int v = lane; v < fresh(ap)->M; v += 64) vscs[v] = IMPF;
@@ -432,11 +502,10 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
     const int64_t bc = min(fresh(ap)->j0, ac + fresh(ap)->chunk_len - 1);
     const int64_t s0 = max(fresh(ap)->i0, ac - (int64_t) fresh(ap)->W + 1);
     const int R = fresh(ap)->R;
-    T *row1 = row0 + R;
 
     {
       const T *rtmpl = (const T *) fresh(ap)->row_tmpl;
-      for (int x = lane; x < R; x += 64) { T t = rtmpl[x]; row0[x] = t; row1[x] = t; }
+      for (int x = lane; x < R; x += 64) { T t = rtmpl[x]; rows[x] = t; rows[R + x] = t; rows[2 * R + x] = t; rows[3 * R + x] = t; }
       const T *dtmpl = (const T *) fresh(ap)->deck_tmpl;
       const int RB = fresh(ap)->RB;
       for (int x = lane; x < RB; x += 64) deck[x] = dtmpl[x];
@@ -446,84 +515,128 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
     float   best = IMPF;
     int64_t envi = INT64_MAX, envj = INT64_MIN;
 
-    for (int64_t j = s0; j <= bc; j++) {
-      const int jp   = (int) (j - s0 + 1);
-      T *cur = (j & 1) ? row1 : row0;
-      T *prv = (j & 1) ? row0 : row1;
-      const uint8_t *dsq_j = fresh(ap)->dsq + (j + 1 - fresh(ap)->dsq_off);   /* left residue of cell d: dsq_j[-d] */
-      const int xj = __builtin_amdgcn_readfirstlane((int) dsq_j[-1]);
+    for (int64_t j0 = s0; j0 <= bc; j0 += 2) {
+      const bool two = j0 + 1 <= bc;                 /* row j0+1 is in this chunk */
+      const int jp0 = (int) (j0 - s0 + 1), jp1 = jp0 + 1;
+      T *cur0 = rows + (size_t) ((j0 + 0) & 3) * R;
+      T *cur1 = rows + (size_t) ((j0 + 1) & 3) * R;
+      T *prv0 = rows + (size_t) ((j0 + 3) & 3) * R;   /* row j0-1 */
+      const uint8_t *dsq_j = fresh(ap)->dsq + (j0 + 1 - fresh(ap)->dsq_off);
+      const int xj0 = __builtin_amdgcn_readfirstlane((int) dsq_j[-1]);
+      const int xj1 = two ? __builtin_amdgcn_readfirstlane((int) dsq_j[0]) : 0;
       const int rdn = fresh(ap)->root_dn;
-      const int rdx = min(jp, fresh(ap)->root_dx);
+      const int rdx0 = min(jp0, fresh(ap)->root_dx);
+      const int rdx1 = two ? min(jp1, fresh(ap)->root_dx) : rdn - 1;
+      const bool vs0 = j0 >= ac, vs1 = two && j0 + 1 >= ac;
       const int local_begin = fresh(ap)->local_begin;
       const uint32_t el = fresh(ap)->el_selfsc;
       const DState *stp = fresh(ap)->st;
       const T *esc = (const T *) fresh(ap)->esc;
       const int nst = fresh(ap)->nst;
+      const int dsq_w = fresh(ap)->W + 64;
+      const uint8_t *dsq_b0 = dsq_j - dsq_w;   /* left residue of cell d of row j0: dsq_b0[dsq_w - d] */
 
       if (local_begin) {
-        for (int d = rdn + lane; d <= rdx; d += 64) { acc_sc[d - rdn] = R::neg_sentinel(); acc_r[d - rdn] = 0; }
+        for (int d = rdn + lane; d <= rdx0; d += 64) acc0[d - rdn] = R::neg_sentinel();
+        for (int d = rdn + lane; d <= rdx1; d += 64) acc1[d - rdn] = R::neg_sentinel();
       }
-
-      const int dsq_w = fresh(ap)->W + 64;
-      const uint8_t *dsq_b = dsq_j - dsq_w;   /* left residue of cell d: dsq_b[dsq_w - d] */
 
       for (int i = 0; i < nst; i++) {
         const DState s = load_state(stp + i);
         const uint32_t hdr = s.hdr;
         const int kind = hdr & 7;
         const int dn = s.dnx & 0xffff;
-        const int dx = min(jp, (int) (s.dnx >> 16));
-        if (dn > dx) continue;
+        const int sdx = (int) (s.dnx >> 16);
+        const int dx0 = min(jp0, sdx);
+        const int dx1 = two ? min(jp1, sdx) : dn - 1;
+        if (dn > max(dx0, dx1)) continue;
         const int v = hdr >> 16;
-        const bool has_begin = local_begin && ((hdr >> 11) & 1);
+        const bool hb = local_begin && ((hdr >> 11) & 1);
         const int cnum = (hdr >> 8) & 7;
         const int em = (hdr >> 3) & 3;
-        const int bdn = max(rdn, dn), bdx = min(rdx, dx);
-        /* emission row: lane y <- score of residue code y (MP: left code y, right code xj) */
-        int ev = 0;
+        const int bdx0 = min(rdx0, dx0), bdx1 = min(rdx1, dx1);
+        /* emission rows: lane y <- score of residue code y (MP: left code y, right code xj) */
+        int ev0 = 0, ev1 = 0;
         if (em != EMITNONE) {
           const int yy = lane < KP ? lane : 0;
-          ev = R::tob(esc[s.aux + (em == EMITPAIR ? yy * KP + xj : yy)]);
+          ev0 = R::tob(esc[s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy)]);
+          ev1 = (em == EMITPAIR) ? R::tob(esc[s.aux + yy * KP + xj1]) : ev0;
         }
+        const bool rd = (hdr >> 7) & 1;
+        const T *crow0 = rd ? prv0 : cur0;
+        const T *crow1 = rd ? cur0 : cur1;
         float vmax;
         if (kind == KIND_GENERIC) {
+          /* IR reads its own row j cells for row j+1: finish row j first */
+          const bool seq = (hdr >> 14) & 1;
+          const int gx1 = seq ? dn - 1 : dx1;
           switch (cnum) {
-          case 2:  vmax = gen_state<S, VSC, 2, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 3:  vmax = gen_state<S, VSC, 3, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 4:  vmax = gen_state<S, VSC, 4, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 5:  vmax = gen_state<S, VSC, 5, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          default: vmax = gen_state<S, VSC, 6, false>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+#define GEN(CN) vmax = gen_state2<S, VSC, CN, false>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+          case 2:  GEN(2); break;
+          case 3:  GEN(3); break;
+          case 4:  GEN(4); break;
+          case 5:  GEN(5); break;
+          default: GEN(6); break;
+#undef GEN
+          }
+          if (seq && dx1 >= dn) {
+            wave_sync();
+            float vm1;
+            switch (cnum) {
+#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+            case 2:  GEN(2); break;
+            case 3:  GEN(3); break;
+            case 4:  GEN(4); break;
+            case 5:  GEN(5); break;
+            default: GEN(6); break;
+#undef GEN
+            }
+            vmax = fmaxf(vmax, vm1);
           }
         } else if (kind == KIND_IL) {
           switch (cnum) {
-          case 2:  vmax = il_state<S, VSC, 2>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 3:  vmax = il_state<S, VSC, 3>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 4:  vmax = il_state<S, VSC, 4>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 5:  vmax = il_state<S, VSC, 5>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          default: vmax = il_state<S, VSC, 6>(CTX_ARGS, s, cur, dn, dx, has_begin, bdn, bdx, v, ev); break;
+#define ILS(CN) vmax = il_state2<S, VSC, CN>(SW_ARGS, s, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, vs0, vs1)
+          case 2:  ILS(2); break;
+          case 3:  ILS(3); break;
+          case 4:  ILS(4); break;
+          case 5:  ILS(5); break;
+          default: ILS(6); break;
+#undef ILS
           }
         } else if (kind == KIND_BEGL) {
-          T *out = deck + (jp & (s.aux & 0xffff)) * (int) (s.aux >> 16);
+          const int rmask = s.aux & 0xffff, rstride = (int) (s.aux >> 16);
+          T *out0 = deck + (jp0 & rmask) * rstride;
+          T *out1 = deck + (jp1 & rmask) * rstride;
           switch (cnum) {
-          case 1:  vmax = gen_state<S, VSC, 1, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 2:  vmax = gen_state<S, VSC, 2, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 3:  vmax = gen_state<S, VSC, 3, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 4:  vmax = gen_state<S, VSC, 4, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          case 5:  vmax = gen_state<S, VSC, 5, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
-          default: vmax = gen_state<S, VSC, 6, true>(CTX_ARGS, s, out, dn, dx, has_begin, bdn, bdx, v, ev); break;
+#define BGL(CN) vmax = gen_state2<S, VSC, CN, true>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
+          case 1:  BGL(1); break;
+          case 2:  BGL(2); break;
+          case 3:  BGL(3); break;
+          case 4:  BGL(4); break;
+          case 5:  BGL(5); break;
+          default: BGL(6); break;
+#undef BGL
           }
         } else {
-          vmax = b_state<S, VSC>(CTX_ARGS, s, cur, deck, dn, dx, has_begin, bdn, bdx, v);
+          vmax = b_state<S, VSC>(tbl, el, lane, acc0, rdn, s, cur0, deck, jp0, dn, dx0, hb, bdx0, vs0);
+          wave_sync();
+          if (two) vmax = fmaxf(vmax, b_state<S, VSC>(tbl, el, lane, acc1, rdn, s, cur1, deck, jp1, dn, dx1, hb, bdx1, vs1));
         }
         if (VSC) {
           for (int o = 32; o >= 1; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-          if (lane == 0 && j >= ac) vscs[v] = fmaxf(vscs[v], vmax);
+          if (lane == 0 && (vs0 || vs1)) vscs[v] = fmaxf(vscs[v], vmax);
         }
         wave_sync();
       } /* states */
 
-      /* ROOT_S (src/cm_dpsearch.c:562-623) and hit candidates */
-      if (j >= ac) {
+      /* ROOT_S (src/cm_dpsearch.c:562-623) and hit candidates, row by row */
+      for (int t = 0; t < (two ? 2 : 1); t++) {
+        const int64_t j = j0 + t;
+        if (j < ac) continue;
+        const int jp = jp0 + t;
+        const T *cur = t ? cur1 : cur0;
+        const T *acc = t ? acc1 : acc0;
+        const int rdx = t ? rdx1 : rdx0;
         CArgs *aq = fresh(ap);
         const int root_cnum = aq->root_cnum;
         const int cand_mode = aq->cand_mode;
@@ -533,14 +646,14 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           const int d = d0 + lane;
           const bool act = d <= rdx;
           float bs = IMPF;
-          int   br = 0;
+          bool lb = false;
           if (act) {
             T al = R::clamp_root(R::add(cur[(uint32_t) (aq->root_child_base[0] + d)], R::bits(aq->root_tsc[0])));
             for (int k = 1; k < root_cnum; k++)
               al = R::op(al, R::add(cur[(uint32_t) (aq->root_child_base[k] + d)], R::bits(aq->root_tsc[k])), tbl);
             if (local_begin) {
-              const T acv = acc_sc[d - rdn];
-              if (acv > al) { al = acv; br = acc_r[d - rdn]; }
+              const T acv = acc[d - rdn];
+              if (acv > al) { al = acv; lb = true; }
             }
             bs = R::score(al);
             best = fmaxf(best, bs);
@@ -567,6 +680,8 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           }
           const unsigned long long m = __ballot(rep);
           if (m) {
+            int br = 0;
+            if (rep && lb) br = begin_argmax<S>(stp, nst, cur, jp, d);
             const int n = __popcll(m);
             unsigned long long basei = 0;
             if (lane == 0) basei = atomicAdd(aq->ncand, (unsigned long long) n);
@@ -583,7 +698,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         }
       }
       wave_sync();
-    } /* j */
+    } /* j0 */
 
     for (int o = 32; o >= 1; o >>= 1) {
       best = fmaxf(best, __shfl_xor(best, o, 64));
@@ -629,7 +744,7 @@ static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds)
 /* LDS bytes of a workgroup of `wpb` wavefronts */
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n)
 {
-  return (semi != SEMI_CYK && tbl_lds ? (size_t) LOGSUM_N * 4 : 0) + (size_t) wpb * acc_n * 8;
+  return (semi != SEMI_CYK && tbl_lds ? (size_t) LOGSUM_N * 4 : 0) + (size_t) wpb * acc_n * 8;   /* two rows of 4-byte maxima */
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
