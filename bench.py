@@ -41,13 +41,11 @@ def log(*a):
 
 
 def ops_per_res(cm, qdbidx, inside):
-    """Algorithmic VALU ops per residue at steady state (bands at jp >= W)."""
-    cells, ops = cm.cells_per_res(qdbidx)
-    if not inside:
-        return ops
-    # each max becomes an FLogsum: ~8 VALU ops + 1 table read instead of 1 op;
-    # per child term 1 add + 1 logsum
-    return ops + (ops - cells) / 2.0 * 7.0
+    """Algorithmic VALU ops per residue at steady state (bands at jp >= W):
+    CYK 2*cnum (+1 emitting) per non-B cell and 2 per bifurcation split;
+    Inside counts every max reduction as an 8-op log-sum."""
+    cells, ops, red = cm.work_per_res(qdbidx)
+    return ops + 7.0 * red if inside else ops
 
 
 def cpu_baseline(cm, codes, threads, sample_len, W):
@@ -100,10 +98,23 @@ def main():
     ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "rnaseP-eubact.c.cm"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=3000, help="residues per CPU-baseline window")
+    ap.add_argument("--cpu-sample", type=int, default=8000, help="residues per CPU-baseline window")
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe > 0:
+        import torch
+        nbytes = int(args.copy_probe * (1 << 30))
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one wavefront per sequence chunk on a single MI355X); "
@@ -168,10 +179,20 @@ def main():
     q, inside = (1, False) if dom == "cyk" else (2, True)
     ops = ops_per_res(cm, q, inside) * n * args.steps
     achieved = ops / (kern[dom] / 1e3) / 1e12
+    kname = {"cyk": "std_cyk_kernel<false>", "finside": "std_inside_lds_kernel<1, false>"}
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": "std_scan_kernel<%s>" % dom,
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": kname[dom],
             "kernel_ms_per_launch": round(kern[dom] / args.steps, 1),
             "gcells_per_s_kernel": {k: round(cells_by[k] / (kern[k] / 1e3) / 1e9, 2) for k in kern if kern[k] > 0}}
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (see profiles/)
+    tj = os.path.join(ROOT, "profiles", "traffic_%s_%dmb.json" % (dom, int(round(n / 1e6))))
+    if os.path.exists(tj):
+        with open(tj) as f:
+            t = json.load(f)
+        if t.get("cm") == cm.c.name.decode():
+            roof["traffic"] = t["hbm_bytes_per_launch"]
+            roof["traffic_note"] = t.get("note", "")
 
     cpu = None
     if not args.no_cpu:
@@ -186,7 +207,7 @@ def main():
                "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4)}
 
     out = {
-        "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU",
+        "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
         "value": round(value, 3),
         "unit": "Gcells/s",
         "n_gpus": 1,
@@ -204,6 +225,7 @@ def main():
                    "cells_per_step": cells_total / args.steps, "hits": hits},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_gate_speedup": None,
     }
     print(json.dumps(out))
 

@@ -353,6 +353,10 @@ int cm_hit_AllowTruncation(CM_t *cm, int pass_idx, int64_t start, int64_t stop, 
  * reference's definition), for forecasts.  Also: this library's cell count,
  * the number of (v,d) cells inside the band at steady state (jp >= W). */
 int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells_per_res, double *ret_ops_per_res);
+/* As infx_CountSearchCells, plus the max (CYK) / log-sum (Inside) reductions
+ * per residue: cnum per non-B cell, kmax-kmin+1 per B cell.  No counterpart
+ * in the reference (the Inside roofline counts each reduction as 8 ops). */
+int infx_CountSearchWork(CM_t *cm, int qdbidx, int truncated, double *ret_cells, double *ret_ops, double *ret_red);
 
 /* Batched truncated scans (no counterpart in the reference, which makes one
  * RefTrCYKScan / RefITrInsideScan call per window, src/cm_pipeline.c:4187-4195):

@@ -105,6 +105,7 @@ _lib.infx_gpu_available.argtypes = [C.c_char_p]
 _lib.infx_last_call_timing.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int)]
 _lib.infx_set_launch_params.argtypes = [C.c_int, C.c_int]
 _lib.infx_CountSearchCells.argtypes = [_CMP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+_lib.infx_CountSearchWork.argtypes = [_CMP, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
 _lib.FLogsum.argtypes = [C.c_float, C.c_float]
 _lib.FLogsum.restype = C.c_float
 _lib.ILogsum.argtypes = [C.c_int, C.c_int]
@@ -239,6 +240,15 @@ class CM:
         if st != eslOK:
             raise ScanError(st, "count failed")
         return cells.value, ops.value
+
+    def work_per_res(self, qdbidx, truncated=False):
+        """(cells, CYK ops, reductions) per residue at steady state: Inside ops
+        count each reduction as one 8-op log-sum instead of one max."""
+        cells, ops, red = C.c_double(), C.c_double(), C.c_double()
+        st = _lib.infx_CountSearchWork(self._p, qdbidx, 1 if truncated else 0, C.byref(cells), C.byref(ops), C.byref(red))
+        if st != eslOK:
+            raise ScanError(st, "count failed")
+        return cells.value, ops.value, red.value
 
 
 def _hits_list(th):

@@ -265,7 +265,15 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
   int status;
 
   if (greedy) {
-    CM_TOPHITS *tmp = cm_tophits_Create();
+    /* ReportHitsGreedily per j, then the overlap removal of the scan call
+     * (src/cm_dpsearch.c:651-661) on compact records: sorted by score (desc),
+     * start (asc) and creation order -- the order of the stable sort in
+     * cm_tophits_SortForOverlapRemoval, since every hit of one call has the
+     * same cm, sequence and strand -- and kept unless they overlap a kept hit
+     * (a bitmap of kept positions).  Only the survivors become CM_HITs, built
+     * and cloned exactly as cm_tophits_CloneHitMostly would. */
+    struct CHit { int64_t start, stop; float score; int root; int mode; int64_t idx; };
+    std::vector<CHit> hs;
     int64_t c = 0;
     while (c < ncand) {
       int64_t j = cand[c].j, e = c;
@@ -285,22 +293,41 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
             do_report = (hit_sc > max_sc_reported && hit_sc >= cutoff) ? 1 : 0;
           }
           if (do_report) {
-            CM_HIT *hit;
-            if ((status = cm_tophits_CreateNextHit(tmp, &hit)) != eslOK) { cm_tophits_Destroy(tmp); return infx_fail(status, errbuf, "out of memory"); }
-            hit->start = i; hit->stop = j; hit->root = cand[k].r;
-            hit->mode = (char) cand[k].mode; hit->score = hit_sc; hit->hmmonly = 0;
+            hs.push_back(CHit{i, j, hit_sc, cand[k].r, cand[k].mode, (int64_t) hs.size()});
             max_sc_reported = hit_sc;
           }
         }
       }
       c = e;
     }
-    for (uint64_t h = 0; h < tmp->N; h++) tmp->unsrt[h].srcL = j0;
-    cm_tophits_SortForOverlapRemoval(tmp);
-    if ((status = cm_tophits_RemoveOrMarkOverlaps(tmp, 0, errbuf)) != eslOK) { cm_tophits_Destroy(tmp); return status; }
+    std::sort(hs.begin(), hs.end(), [](const CHit &a, const CHit &b) {
+      if (a.score != b.score) return a.score > b.score;
+      if (a.start != b.start) return a.start < b.start;
+      return a.idx < b.idx;
+    });
+    const int64_t L = j0 - i0 + 1;
+    std::vector<uint64_t> used((size_t) (L + 64) / 64 + 1, 0);
+    auto any_used = [&](int64_t lo, int64_t hi) {        /* positions lo..hi, 0-based in the call's range */
+      for (int64_t x = lo; x <= hi; ) {
+        if ((x & 63) == 0 && x + 63 <= hi) { if (used[x >> 6]) return true; x += 64; }
+        else { if (used[x >> 6] >> (x & 63) & 1ull) return true; x++; }
+      }
+      return false;
+    };
+    CM_TOPHITS *tmp = cm_tophits_Create();
+    for (const CHit &h : hs) {
+      const int64_t lo = h.start - i0, hi = h.stop - i0;
+      if (any_used(lo, hi)) continue;
+      for (int64_t x = lo; x <= hi; x++) used[x >> 6] |= 1ull << (x & 63);
+      CM_HIT *hit;
+      if ((status = cm_tophits_CreateNextHit(tmp, &hit)) != eslOK) { cm_tophits_Destroy(tmp); return infx_fail(status, errbuf, "out of memory"); }
+      hit->start = h.start; hit->stop = h.stop; hit->root = h.root;
+      hit->mode = (char) h.mode; hit->score = h.score; hit->hmmonly = 0; hit->srcL = j0;
+    }
+    for (uint64_t h = 0; h < tmp->N; h++) tmp->hit[h] = tmp->unsrt + h;   /* already in sorted order */
+    tmp->is_sorted_for_overlap_removal = 1;
     for (uint64_t h = 0; h < tmp->N; h++)
-      if (!(tmp->hit[h]->flags & CM_HIT_IS_REMOVED_DUPLICATE))
-        if ((status = cm_tophits_CloneHitMostly(tmp, (int) h, hitlist)) != eslOK) { cm_tophits_Destroy(tmp); return infx_fail(status, errbuf, "problem copying hit to hitlist, out of memory?"); }
+      if ((status = cm_tophits_CloneHitMostly(tmp, (int) h, hitlist)) != eslOK) { cm_tophits_Destroy(tmp); return infx_fail(status, errbuf, "problem copying hit to hitlist, out of memory?"); }
     cm_tophits_Destroy(tmp);
     return eslOK;
   }

@@ -920,13 +920,13 @@ extern "C" int cm_Configure(CM_t *cm, char *errbuf, int W_from_cmdline)
  * residue: non-B states 2*cnum (+1 if emitting) per cell, B states
  * 2*(kmax-kmin+1) per cell, E states excluded.  The root state v=0 counts
  * its own children only. */
-extern "C" int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells, double *ret_ops)
+extern "C" int infx_CountSearchWork(CM_t *cm, int qdbidx, int truncated, double *ret_cells, double *ret_ops, double *ret_red)
 {
   if (!cm || qdbidx < 0 || qdbidx > 2) return eslEINVAL;
   const int W = cm->W;
   const int *dmin = qdbidx == 1 ? cm->qdbinfo->dmin1 : qdbidx == 2 ? cm->qdbinfo->dmin2 : NULL;
   const int *dmax = qdbidx == 1 ? cm->qdbinfo->dmax1 : qdbidx == 2 ? cm->qdbinfo->dmax2 : NULL;
-  double cells = 0, ops = 0;
+  double cells = 0, ops = 0, red = 0;
   for (int v = 0; v < cm->M; v++) {
     int st = cm->sttype[v];
     if (st == E_st) continue;
@@ -954,13 +954,20 @@ extern "C" int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double
           kmin = ESL_MAX(0, d - dx_w);
           kmax = ESL_MIN(dx_y, d);
         } else { kmin = 0; kmax = d; }
-        if (kmax >= kmin) ops += 2.0 * (kmax - kmin + 1);
+        if (kmax >= kmin) { ops += 2.0 * (kmax - kmin + 1); red += kmax - kmin + 1; }
       }
     } else {
       ops += (double) (dx - dn + 1) * (2 * cm->cnum[v] + (Emitmode(st) != EMITNONE ? 1 : 0));
+      red += (double) (dx - dn + 1) * cm->cnum[v];
     }
   }
+  if (ret_red) *ret_red = red;
   if (ret_cells) *ret_cells = cells;
   if (ret_ops)   *ret_ops = ops;
   return eslOK;
+}
+
+extern "C" int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells, double *ret_ops)
+{
+  return infx_CountSearchWork(cm, qdbidx, truncated, ret_cells, ret_ops, NULL);
 }

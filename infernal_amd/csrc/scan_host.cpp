@@ -579,7 +579,11 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
       (status = D->d_chunk.reserve((size_t) nchunks * 24 + 64)) != eslOK ||
       (status = D->d_ncand.reserve(64)) != eslOK)
     return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
-  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
+  /* candidate buffer: four per residue (the greedy record filter keeps a
+   * handful per j), at most 8 GB; an overflow re-runs the scan once with the
+   * exact size */
+  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 4 * L, ((int64_t) 8 << 30) / (int64_t) sizeof(Cand)));
+  cand_cap = ESL_MAX(cand_cap, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
   if ((status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   /* target residues i0..j0 behind and before 64 zero bytes (residue codes
@@ -691,6 +695,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     *ret_vsc = vsc;
   }
 
+  auto t_resolve = std::chrono::steady_clock::now();
   if (hitlist) {
     std::vector<INFX_CAND> cands(ncand);
     for (size_t k = 0; k < ncand; k++) { cands[k].j = hc[k].j; cands[k].d = hc[k].d; cands[k].sc = hc[k].sc; cands[k].r = hc[k].r; cands[k].mode = TRMODE_J; }
@@ -702,6 +707,10 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   double cells = 0;
   infx_count_scan_cells(cm, smx, qdbidx, 0, i0, j0, &cells);
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (getenv("INFX_VERBOSE"))
+    fprintf(stderr, "[infx] %s: wall %.1f ms = kernel %.1f ms (%d launch(es)) + hit replay %.1f ms (%llu candidates) + rest\n", fname,
+            wall, kernel_ms_total, launches,
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_resolve).count(), (unsigned long long) ncand);
   infx_timing_add(wall, kernel_ms_total, (int64_t) cells, launches);
   return eslOK;
 }
