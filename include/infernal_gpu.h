@@ -34,6 +34,7 @@ extern "C" {
 #define eslEINCOMPAT   10
 #define eslEINVAL      11
 #define eslESYS        12
+#define eslEINCONCEIVABLE 14
 #define eslERANGE      16
 #define eslENODEVICE  100   /* this library only: no usable GPU */
 #define eslERRBUFSIZE 128
@@ -390,6 +391,18 @@ void infx_last_call_timing(double *ret_wall_ms, double *ret_kernel_ms, int64_t *
  * arithmetic on the host. */
 int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int64_t L, void *out_st, int cap,
                            int *ret_R, int *ret_RB, int *ret_nesc, int *ret_root, char *errbuf);
+/* Query-dependent bands and W from the model's transition probabilities
+ * (local ends off, standard local begins on).  Fills qdbinfo->dmin1/dmax1
+ * (beta1) and dmin2/dmax2 (beta2) for every state and *ret_W (beta_W).
+ *   infx_CalculateQueryDependentBands  replaces CalculateQueryDependentBands
+ *                                      (src/cm_qdband.c:45): Z = 4*clen,
+ *                                      doubled until the tail is negligible
+ *   infx_BandCalculationEngine         replaces BandCalculationEngine
+ *                                      (src/cm_qdband.c:163) for one Z;
+ *                                      eslERANGE if Z is too small */
+int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W);
+int infx_CalculateQueryDependentBands(CM_t *cm, char *errbuf, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W, int *ret_Z);
+
 /* Override the work split (0 = automatic): residues per chunk and waves per
  * block.  Used by the autotuner and tests. */
 void infx_set_launch_params(int chunk_len, int waves_per_block);
