@@ -78,6 +78,50 @@ def cpu_baseline(cm, codes, threads, sample_len, W):
     return res
 
 
+def cyk_1500(args, ia, synth, threads):
+    """Bit-exact FastCYKScan (QDB1) of the 1,500-nt synthetic model over
+    --cyk1500-mb Mb of the same kind of target, on the GPU and (threads not
+    None) with the oracle on the host, for the GPU/CPU Gcells/s ratio."""
+    path = os.path.join(ROOT, "tests", "golden", "cm", "synth_nested1500.cm")
+    cm = ia.CM.read(path)
+    cm.configure(local=True)
+    n = int(args.cyk1500_mb * 1e6)
+    bg = synth.markov2_background(n, gc=0.40, seed=1500)
+    frags = synth.homologs(ia.CM.read(path), max(1, n // 20000), seed=15)
+    codes, _ = synth.plant(bg, frags, seed=16)
+    dsq = ia.dsq_from_codes(codes)
+    ia.FastCYKScan(cm, 1, dsq, 1, min(n, 200000), 0.0, want_hits=True)
+    t0 = time.perf_counter()
+    r = ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
+    wall = time.perf_counter() - t0
+    tm = ia.last_call_timing()
+    log("1500-nt CYK: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (wall, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
+    out = {"cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), cm.clen, cm.M, cm.W),
+           "target_mb": n / 1e6, "homologs": len(frags), "hits": len(r["hits"]),
+           "gpu_gcells_per_s": round(tm["cells"] / wall / 1e9, 3),
+           "gpu_kernel_gcells_per_s": round(tm["cells"] / (tm["kernel_ms"] / 1e3) / 1e9, 3)}
+    if threads:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind as ob
+        ob.load()
+        Lw = 2000 + cm.W - 1
+        starts = np.random.default_rng(98).integers(1, n - Lw, threads)
+
+        def one(s):
+            ob.scan(cm, ob.ORACLE_CYK, 1, dsq, int(s), int(s) + Lw - 1, 1e30, want_rows=False)
+
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, starts))
+        cw = time.perf_counter() - t0
+        cells = sum(_cells(cm, 1, int(s), int(s) + Lw - 1) for s in starts)
+        out["cpu_gcells_per_s"] = round(cells / cw / 1e9, 4)
+        out["cpu_cores"] = threads
+        out["cpu_sample"] = "oracle FastCYKScan, %d windows of %d nt in %d threads, %.1f s" % (threads, Lw, threads, cw)
+        out["gpu_over_cpu"] = round(out["gpu_gcells_per_s"] / out["cpu_gcells_per_s"], 1)
+    return out
+
+
 def _cells(cm, qdbidx, i0, j0):
     import ctypes as C
     import infernal_amd as ia
@@ -95,12 +139,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mb", type=float, default=100.0, help="target length in megabases")
     ap.add_argument("--homologs", type=int, default=2000)
-    ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "rnaseP-eubact.c.cm"))
+    ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "synth_multiloop350.cm"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=8000, help="residues per CPU-baseline window")
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--cyk1500-mb", type=float, default=10.0,
+                    help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -206,6 +252,10 @@ def main():
                "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
                "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4)}
 
+    big = None
+    if args.cyk1500_mb > 0:
+        big = cyk_1500(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
+
     out = {
         "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
         "value": round(value, 3),
@@ -226,6 +276,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned_gate_speedup": None,
+        "cyk_1500nt": big,
     }
     print(json.dumps(out))
 

@@ -4,12 +4,14 @@ Bar: FastCYKScan and FastIInsideScan bit-exact (scores, hit lists, envelopes,
 per-state maxima); FastFInsideScan within 0.01 bits (it is bit-exact in
 practice, because the GPU follows the same FLogsum order and table).
 """
+import os
+
 import numpy as np
 import pytest
 
 import infernal_amd as ia
 import oracle_bind as ob
-from helpers import load_cm, random_dsq, hits_equal, describe_hit_diff
+from helpers import CMDIR, load_cm, random_dsq, hits_equal, describe_hit_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -106,3 +108,31 @@ def test_bifurcating_models():
         dsq = random_dsq(2500, seed=51)
         _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 2500, 0.0, vsc=True)
         _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 1500, 0.0)
+
+
+def _planted(cm_name, L, nhom, seed):
+    """Markov background (40% GC) with emitted homologs of the named model."""
+    from infernal_amd import synth
+    frags = synth.homologs(ia.CM.read(os.path.join(CMDIR, cm_name)), nhom, seed=seed)
+    bg = synth.markov2_background(L, gc=0.40, seed=seed + 1)
+    codes, _ = synth.plant(bg, frags, seed=seed + 2)
+    return ia.dsq_from_codes(codes)
+
+
+@pytest.mark.parametrize("name", ["synth_cloverleaf76.cm", "synth_multiloop350.cm"])
+def test_synthetic_models(name):
+    """The benchmark's synthetic models on targets with planted homologs:
+    CYK (QDB1) hit lists bit-exact, FInside (QDB2) within 0.01 bits."""
+    cm = load_cm(name)
+    dsq = _planted(name, 6000, 4, seed=21)
+    g, _ = _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 6000, 0.0, env=5.0, vsc=True)
+    assert g["sc"] > 20.0 and len(g["hits"]) >= 2
+    _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 4000, 0.0, tol=FINSIDE_TOL)
+
+
+def test_synthetic_1500nt_cyk():
+    """The 1,500-nt model (W 1522): CYK over a target spanning several chunks."""
+    cm = load_cm("synth_nested1500.cm")
+    dsq = _planted("synth_nested1500.cm", 5000, 2, seed=31)
+    g, _ = _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 5000, 0.0, vsc=True)
+    assert g["sc"] > 50.0
