@@ -1,24 +1,24 @@
-"""Covariance-model construction from a consensus structure and an alignment
-(a restatement of the parts of cmbuild this package needs to make the
-benchmark and test models).
+"""Covariance-model construction from an alignment and its consensus
+structure, with cmbuild's default options (a restatement of the parts of
+src/cmbuild.c, src/cm_modelmaker.c, src/cm_parsetree.c, src/eweight.c and
+src/prior.c on that path), used to make the benchmark and test models.
 
-Topology (exact):
-  guide_tree(ct)      HandModelmaker's guide-tree construction
-                      (src/cm_modelmaker.c:104-430) for an alignment whose
-                      columns are all consensus: ROOT, then per subsequence
-                      i..j: END if empty, MATL if i unpaired, MATR if j
-                      unpaired, MATP if i pairs j, else BIF split at the k
-                      (stepping over closed stems from ct[i]+1) that best
-                      balances the two sides; BEGL i..k-1 and BEGR k..j.
-  states(nodes)       cm_from_guide (src/cm_modelmaker.c:596-870): the states
-                      of each node type and their cfirst/cnum/plast/pnum.
+  guide_tree(ct, mat)  HandModelmaker's guide tree (src/cm_modelmaker.c:203-425)
+  states(root)         cm_from_guide (src/cm_modelmaker.c:596-870)
+  rebalance(layout)    CMRebalance state numbering (src/cm.c:1821-1990)
+  pb_weights           Easel's position-based weights as cmbuild calls them
+  match_columns        AssignMatchColumnsForMsa (src/cm_modelmaker.c:511-560)
+  count_parse          Transmogrify + ParsetreeCountExceptTruncatedMPs
+  entropy_weight       cm_EntropyWeight (src/eweight.c:120-230)
+  priorify             PriorifyCM with the Dirichlet mixtures of a prior file
+  detach_dual_inserts  cm_find_and_detach_dual_inserts (src/cm_modelmaker.c)
+  build_cm_from_alignment / build_cm   the whole path, bands and W from
+                       infx_CalculateQueryDependentBands, INFERNAL1/a text
 
-Parameters (cmbuild's defaults, restated; see build_cm for what differs):
-  Henikoff position-based sequence weights, counts along each sequence's
-  parse, entropy weighting of the effective sequence number to the
-  0.59 bits/position target, Dirichlet-mixture priors from
-  default.prior-style files, EL self-loop 0.94, pbegin/pend 0.05, and the
-  query-dependent bands and W from infx_CalculateQueryDependentBands.
+Rebuilding the reference's tutorial models (tRNA5, 5S_rRNA, Cobalamin) from
+their alignments reproduces the shipped files: same EFFN, W and bands, every
+score within print rounding.  Not covered: --hand, truncated ('~')
+sequences, non-default weighting, prior or entropy options.
 """
 import math
 
@@ -68,32 +68,61 @@ class Node:
         self.left = self.right = None
 
 
-def guide_tree(ct):
-    """Guide tree for consensus positions 1..clen with pairing ct (HandModelmaker,
-    every column consensus).  Returns the ROOT node."""
-    clen = len(ct) - 1
+def guide_tree(ct, mat=None):
+    """Guide tree (HandModelmaker, src/cm_modelmaker.c:203-425) over alignment
+    columns 1..n with consensus pairing ct (0 = unpaired or non-consensus) and
+    mat[c] True for consensus columns (None: every column).  Per subsequence
+    i..j (both consensus columns): END if empty, MATL if i unpaired, MATR if j
+    unpaired, MATP if i pairs j, else BIF split at the k (stepping over closed
+    stems from ct[i]+1) whose two sides have the closest consensus lengths,
+    BEGL i..k-1 and BEGR k..j.  Non-consensus columns are stepped over; they
+    are the insert columns of the node before them.  Node spans are alignment
+    columns.  Returns the ROOT node."""
+    n = len(ct) - 1
+    if mat is None:
+        mat = [True] * (n + 2)
+    a2c = [0] * (n + 2)
+    c = 0
+    for col in range(1, n + 1):
+        if mat[col]:
+            c += 1
+            a2c[col] = c
+
+    def next_mat(i, j):
+        while i <= j and not mat[i]:
+            i += 1
+        return i
+
+    def prev_mat(j, i):
+        while j >= i and not mat[j]:
+            j -= 1
+        return j
 
     def dummy(i, j):
         if i > j:
             return Node(END_nd, i, j)
         if ct[i] == 0:
             nd = Node(MATL_nd, i, j)
-            nd.left = dummy(i + 1, j)
+            nd.left = dummy(next_mat(i + 1, j), j)
             return nd
         if ct[j] == 0:
             nd = Node(MATR_nd, i, j)
-            nd.left = dummy(i, j - 1)
+            nd.left = dummy(i, prev_mat(j - 1, i))
             return nd
         if ct[i] == j:
             nd = Node(MATP_nd, i, j)
-            nd.left = dummy(i + 1, j - 1)
+            i2 = next_mat(i + 1, j)
+            nd.left = dummy(i2, prev_mat(j - 1, i2))
             return nd
         # bifurcation: the most balanced split point (first one on ties)
         nd = Node(BIF_nd, i, j)
-        bestk, bestdiff = ct[i] + 1, clen + 2
+        bestk, bestdiff = ct[i] + 1, n + 2
         k = ct[i] + 1
         while k <= ct[j]:
-            diff = abs((k - i) - (j - k + 1))
+            kp = k
+            while not mat[kp]:
+                kp += 1
+            diff = abs((a2c[kp] - a2c[i]) - (a2c[j] - a2c[kp] + 1))
             if diff < bestdiff:
                 bestdiff, bestk = diff, k
             while ct[k] == 0:
@@ -102,12 +131,13 @@ def guide_tree(ct):
         bl = Node(BEGL_nd, i, bestk - 1)
         bl.left = dummy(i, bestk - 1)
         br = Node(BEGR_nd, bestk, j)
-        br.left = dummy(bestk, j)
+        br.left = dummy(next_mat(bestk, j), j)
         nd.left, nd.right = bl, br
         return nd
 
-    root = Node(ROOT_nd, 1, clen)
-    root.left = dummy(1, clen)
+    root = Node(ROOT_nd, 1, n)
+    i = next_mat(1, n)
+    root.left = dummy(i, prev_mat(n, i))
     return root
 
 
@@ -318,51 +348,196 @@ class Model:
         return self.nodemap[nd]
 
 
-def henikoff_weights(aln):
-    """Position-based sequence weights (Henikoff & Henikoff): each column
-    gives 1/(r * n_a) to a sequence with residue a, r residue types present,
-    n_a sequences with a; weights sum to the number of sequences."""
-    aln = np.asarray(aln)
-    n, L = aln.shape
-    w = np.zeros(n)
-    for c in range(L):
-        col = aln[:, c]
-        vals, cnt = np.unique(col, return_counts=True)
-        r = len(vals)
-        lut = dict(zip(vals.tolist(), cnt.tolist()))
-        w += np.array([1.0 / (r * lut[x]) for x in col.tolist()])
-    return w * (n / w.sum())
+# digital alignment codes: 0-3 ACGU, GAP, degenerate residues, MISSING ('~')
+GAP = 4
+DEGEN = "RYMKSWHBVDN"
+MISSING = 16
+_DEGEN_SETS = {"R": "AG", "Y": "CU", "M": "AC", "K": "GU", "S": "CG", "W": "AU", "H": "ACU", "B": "CGU",
+               "V": "ACG", "D": "AGU", "N": "ACGU"}
+# fractional canonical counts of each code (Easel's esl_abc_FCount with weight 1)
+FCOUNT = np.zeros((17, 4))
+for _x in range(4):
+    FCOUNT[_x, _x] = 1.0
+for _k, _ch in enumerate(DEGEN):
+    for _r in _DEGEN_SETS[_ch]:
+        FCOUNT[5 + _k, "ACGU".index(_r)] = 1.0 / len(_DEGEN_SETS[_ch])
 
 
-def count_all_match(m, aln, wts):
-    """Counts from ungapped sequences aligned to every consensus column: each
-    sequence follows the all-match parse (MP/ML/MR/B/S/E per node)."""
-    aln = np.asarray(aln)
-    nnodes = len(m.nd_types)
-    t = np.zeros((m.M, 6))
-    e = np.zeros((m.M, 16))
-    # the parse visits each node's first state; transitions go from it to the
-    # first state of the child node (the left child for a bifurcation's BEGL)
-    children = {}
-    order = preorder(m.root)
-    node_of = {id(nd): k for k, (nd, _) in enumerate(order)}
-    for k, (nd, parent) in enumerate(order):
-        if nd.type in (BIF_nd, END_nd):
+def read_stockholm(path):
+    """Aligned sequences (in file order, blocks joined) and #=GC lines of a
+    one-alignment Stockholm file."""
+    seqs, order, gc = {}, [], {}
+    with open(path) as fh:
+        for line in fh:
+            if not line.strip() or line.startswith("# STOCKHOLM") or line.startswith("//"):
+                continue
+            if line.startswith("#=GC"):
+                _, tag, val = line.split(None, 2)
+                gc[tag] = gc.get(tag, "") + val.strip()
+                continue
+            if line.startswith("#"):
+                continue
+            name, seq = line.split()
+            if name not in seqs:
+                seqs[name] = ""
+                order.append(name)
+            seqs[name] += seq
+    return [seqs[n] for n in order], gc
+
+
+def digitize_alignment(aseqs):
+    """Aligned text sequences -> int array of the codes above (T reads as U;
+    '.', '-' and '_' are gaps; unknown letters read as N)."""
+    lut = {"A": 0, "C": 1, "G": 2, "U": 3, "T": 3, ".": GAP, "-": GAP, "_": GAP, "~": MISSING}
+    for k, ch in enumerate(DEGEN):
+        lut[ch] = 5 + k
+    out = np.zeros((len(aseqs), len(aseqs[0])), dtype=np.int64)
+    for r, seq in enumerate(aseqs):
+        out[r] = [lut.get(ch, 5 + DEGEN.index("N")) for ch in seq.upper()]
+    return out
+
+
+def pb_weights(ax, symfrac=0.5):
+    """Henikoff position-based weights as cmbuild computes them by default
+    (Easel's esl_msaweight_PB_adv, ignoring RF): over the columns whose
+    residue fraction (residues / (residues + gaps)) is >= symfrac, a sequence
+    with canonical residue a in a column with r canonical residue types and
+    n_a copies of a gets 1/(r n_a); each sequence's sum is divided by its
+    number of canonical residues in those columns; the weights are scaled to
+    sum to the number of sequences.  (Pinned by the effective sequence
+    numbers and match emissions of the tutorial models.)"""
+    nseq, alen = ax.shape
+    if nseq == 1:
+        return np.ones(1)
+    isres = (ax < GAP) | ((ax > GAP) & (ax < MISSING))
+    denom = isres.sum(0) + (ax == GAP).sum(0)
+    cols = [c for c in range(alen) if denom[c] > 0 and isres[:, c].sum() / denom[c] >= symfrac]
+    w = np.zeros(nseq)
+    for c in cols:
+        col = ax[:, c]
+        canon = col < GAP
+        cnt = np.bincount(col[canon], minlength=4)
+        ntypes = int((cnt > 0).sum())
+        if ntypes == 0:
             continue
-        children[k] = node_of[id(nd.left)]
-    for k, ch in children.items():
-        v = m.nodemap[k]
-        y = m.nodemap[ch]
-        t[v][y - m.cfirst[v]] += wts.sum()
-    for k, (nd, _) in enumerate(order):
-        v = m.nodemap[k]
-        if nd.type == MATP_nd:
-            np.add.at(e[v], aln[:, nd.i - 1] * 4 + aln[:, nd.j - 1], wts)
-        elif nd.type == MATL_nd:
-            np.add.at(e[v], aln[:, nd.i - 1], wts)
-        elif nd.type == MATR_nd:
-            np.add.at(e[v], aln[:, nd.j - 1], wts)
-    return t, e
+        w[canon] += 1.0 / (ntypes * cnt[col[canon]])
+    rlen = (ax[:, cols] < GAP).sum(1)
+    w = np.where(rlen > 0, w / np.maximum(rlen, 1), w)
+    return w * (nseq / w.sum())
+
+
+def henikoff_weights(aln):
+    """pb_weights of an ungapped alignment of canonical residues."""
+    return pb_weights(np.asarray(aln))
+
+
+def match_columns(ax, wts, symfrac=0.5):
+    """AssignMatchColumnsForMsa (src/cm_modelmaker.c:511-560), weighted: a
+    column is consensus if its weighted residue fraction is >= symfrac.
+    Returns mat[0..alen+1] (1-based, False at both ends)."""
+    isres = (ax < GAP) | ((ax > GAP) & (ax < MISSING))
+    w = np.asarray(wts, dtype=np.float32)
+    mat = [False] * (ax.shape[1] + 2)
+    for c in range(ax.shape[1]):
+        r = np.float32(w[isres[:, c]].sum())
+        tot = np.float32(w[isres[:, c] | (ax[:, c] == GAP)].sum())
+        mat[c + 1] = bool(r > 0 and r / tot >= symfrac)
+    return mat
+
+
+# state offsets within a node, in cm_from_guide order
+_OFF = {(ROOT_nd, S_st): 0, (ROOT_nd, IL_st): 1, (ROOT_nd, IR_st): 2, (BIF_nd, B_st): 0,
+        (MATP_nd, MP_st): 0, (MATP_nd, ML_st): 1, (MATP_nd, MR_st): 2, (MATP_nd, D_st): 3, (MATP_nd, IL_st): 4,
+        (MATP_nd, IR_st): 5, (MATL_nd, ML_st): 0, (MATL_nd, D_st): 1, (MATL_nd, IL_st): 2,
+        (MATR_nd, MR_st): 0, (MATR_nd, D_st): 1, (MATR_nd, IR_st): 2, (BEGL_nd, S_st): 0, (BEGR_nd, S_st): 0,
+        (BEGR_nd, IL_st): 1, (END_nd, E_st): 0}
+
+
+def count_parse(m, gorder, row, wgt, t, e):
+    """Transmogrify + ParsetreeCountExceptTruncatedMPs
+    (src/cm_modelmaker.c:973-1400, src/cm_parsetree.c:301-365) for one aligned
+    sequence without missing data: walk the guide tree in preorder; each
+    consensus node uses its match, split or delete state by the gap pattern
+    of its columns, and each residue in its insert columns adds an IL (left
+    inserts, left to right) or IR (right inserts, right to left) step.  Adds
+    wgt to every transition along the parse (B and E have none) and the
+    residues' fractional counts to the emitting states."""
+    w = float(np.float32(wgt))
+    isgap = row == GAP
+    isres = ~isgap & (row != MISSING)
+    node_index = {id(nd): k for k, (nd, _) in enumerate(gorder)}
+    stack = []
+    cur = [None]          # state of the trace node the next one attaches to
+
+    def step(node, sttype, left=None, right=None):
+        z = m.nodemap[node] + _OFF[(m.nd_types[node], sttype)]
+        v = cur[0]
+        if v is not None and m.st[v] not in (B_st, E_st):
+            t[v][z - m.cfirst[v]] += w
+        if sttype == MP_st:
+            e[z] += np.outer(FCOUNT[row[left - 1]], FCOUNT[row[right - 1]]).ravel() * w
+        elif sttype in (ML_st, IL_st):
+            e[z, :4] += FCOUNT[row[left - 1]] * w
+        elif sttype in (MR_st, IR_st):
+            e[z, :4] += FCOUNT[row[right - 1]] * w
+        cur[0] = z
+
+    for node, (nd, _) in enumerate(gorder):
+        typ = nd.type
+        nx = nd.left
+        if typ == ROOT_nd:
+            cur[0] = None
+            step(node, S_st)
+            for i in range(nd.i, nx.i):
+                if isres[i - 1]:
+                    step(node, IL_st, left=i)
+            for j in range(nd.j, nx.j, -1):
+                if isres[j - 1]:
+                    step(node, IR_st, right=j)
+        elif typ == BIF_nd:
+            step(node, B_st)
+            stack.append(cur[0])
+        elif typ == BEGL_nd:
+            step(node, S_st)
+        elif typ == BEGR_nd:
+            cur[0] = None      # right child of the B: no transition counted
+            stack.pop()
+            step(node, S_st)
+            for i in range(nd.i, nx.i):
+                if isres[i - 1]:
+                    step(node, IL_st, left=i)
+        elif typ == END_nd:
+            step(node, E_st)
+        elif typ == MATP_nd:
+            gl, gr = isgap[nd.i - 1], isgap[nd.j - 1]
+            step(node, D_st if (gl and gr) else MR_st if gl else ML_st if gr else MP_st, left=nd.i, right=nd.j)
+            for i in range(nd.i + 1, nx.i):
+                if isres[i - 1]:
+                    step(node, IL_st, left=i)
+            for j in range(nd.j - 1, nx.j, -1):
+                if isres[j - 1]:
+                    step(node, IR_st, right=j)
+        elif typ == MATL_nd:
+            step(node, D_st if isgap[nd.i - 1] else ML_st, left=nd.i)
+            for i in range(nd.i + 1, nx.i):
+                if isres[i - 1]:
+                    step(node, IL_st, left=i)
+        elif typ == MATR_nd:
+            step(node, D_st if isgap[nd.j - 1] else MR_st, right=nd.j)
+            for j in range(nd.j - 1, nx.j, -1):
+                if isres[j - 1]:
+                    step(node, IR_st, right=j)
+
+
+def zero_flanking_insert_counts(m, t):
+    """cm_zero_flanking_insert_counts (src/cm_modelmaker.c:2171-2230): counts
+    through ROOT_IL/ROOT_IR are moved onto ROOT_S's transitions into node 1,
+    and the two inserts keep no counts."""
+    t[0][0] = t[0][1] = 0.0
+    for k in range(m.cnum[0] - 2):
+        t[0][2 + k] += t[1][2 + k] + t[2][1 + k]
+    t[1, :] = 0.0
+    t[2, :] = 0.0
 
 
 def priorify(m, t_counts, e_counts, pri):
@@ -514,42 +689,57 @@ def cm_text(m, t, e, name, W, bands, nseq, effn, el_self=0.94, pbegin=0.05, pend
     return "\n".join(out) + "\n"
 
 
-def build_cm(ss, aln, name, prior_path, el_self=0.94):
-    """A CM from a consensus structure and an ungapped alignment (sequences
-    as residue codes 0..3, one column per consensus position), with cmbuild's
-    default weighting and priors.  Returns (text, info).
-
-    Restated from the reference: guide tree, state layout and numbering
-    (pinned against shipped models), Henikoff position-based weights, counts,
-    entropy weighting to set_target_relent, the Dirichlet-mixture posterior
-    means, dual-insert detaching, flat insert emissions, bands and W.
-    Differences: alignments with gaps (deletes, inserts, fragments) are not
-    handled, and Easel's bisection and weighting code is restated from its
-    documented behaviour (parameters parity unpinned)."""
-    import ctypes as C
-    from . import CM, _lib
-    ct = ss_to_ct(ss)
-    m = Model(ct)
-    aln = np.asarray(aln, dtype=np.int64)
-    nseq = aln.shape[0]
-    if aln.shape[1] != m.clen:
-        raise ValueError("alignment has %d columns, structure %d" % (aln.shape[1], m.clen))
-    pri = read_prior(prior_path)
-    wts = henikoff_weights(aln)
-    tc, ec = count_all_match(m, aln, wts)
+def estimate(ax, ss_cons, pri, symfrac=0.5):
+    """cmbuild's default path from a digital alignment (codes above) and its
+    consensus structure to probabilities: PB weights, weighted consensus
+    columns, structure knocked out of non-consensus columns, guide tree and
+    rebalanced states, per-sequence parse counts, flanking-insert counts
+    zeroed, entropy weighting, Dirichlet-mixture posterior means, dual
+    inserts detached, insert emissions flat.  Sequences must not contain
+    missing data ('~'); local-end columns are not used.
+    Returns (model, t, e, neff, etarget)."""
+    ax = np.asarray(ax, dtype=np.int64)
+    if (ax == MISSING).any():
+        raise ValueError("truncated (~) sequences are not supported")
+    nseq, alen = ax.shape
+    if len(ss_cons) != alen:
+        raise ValueError("structure has %d columns, alignment %d" % (len(ss_cons), alen))
+    wts = pb_weights(ax, symfrac)
+    mat = match_columns(ax, wts, symfrac)
+    ct = ss_to_ct(ss_cons)
+    for col in range(1, alen + 1):
+        if not mat[col]:
+            if ct[col]:
+                ct[ct[col]] = 0
+            ct[col] = 0
+    c2a = [0] + [col for col in range(1, alen + 1) if mat[col]]
+    a2c = {col: c for c, col in enumerate(c2a) if c}
+    ct_cons = [0] + [a2c[ct[col]] if ct[col] else 0 for col in c2a[1:]]
+    m = Model(ct_cons)
+    gorder = preorder(guide_tree(ct, mat))
+    if [nd.type for nd, _ in gorder] != m.nd_types:
+        raise AssertionError("alignment guide tree differs from the consensus one")
+    tc = np.zeros((m.M, 6))
+    ec = np.zeros((m.M, 16))
+    for r in range(nseq):
+        count_parse(m, gorder, ax[r], wts[r], tc, ec)
+    zero_flanking_insert_counts(m, tc)
     nbps = sum(1 for typ in m.nd_types if typ == MATP_nd)
     etarget = target_relent(m.clen, nbps)
     neff = entropy_weight(m, tc, ec, nseq, pri, etarget)
-    s = neff / nseq
-    t, e = priorify(m, tc * s, ec * s, pri)
+    scale = float(np.float32(neff / nseq))
+    t, e = priorify(m, tc * scale, ec * scale, pri)
     t = detach_dual_inserts(m, t)
     for v in range(m.M):
         if m.st[v] in (IL_st, IR_st):
             e[v, :] = 0.0
             e[v, :4] = 0.25
-    zero = [0] * m.M
-    full = [4 * m.clen] * m.M
-    text0 = cm_text(m, t, e, name, 4 * m.clen, (zero, full, zero, full), nseq, neff, el_self)
+    return m, t, e, neff, etarget
+
+
+def _bands_and_W(text0, M):
+    import ctypes as C
+    from . import CM, _lib
     cm = CM.from_text(text0)
     W = C.c_int()
     Z = C.c_int()
@@ -561,12 +751,37 @@ def build_cm(ss, aln, name, prior_path, el_self=0.94):
     if st != 0:
         raise RuntimeError(eb.value.decode())
     q = cm.c.qdbinfo.contents
-    bands = ([q.dmin1[v] for v in range(m.M)], [q.dmax1[v] for v in range(m.M)],
-             [q.dmin2[v] for v in range(m.M)], [q.dmax2[v] for v in range(m.M)])
-    text = cm_text(m, t, e, name, W.value, bands, nseq, neff, el_self)
-    info = {"clen": m.clen, "M": m.M, "nodes": len(m.nd_types), "W": W.value, "neff": neff, "etarget": etarget,
-            "nbps": nbps, "nbif": sum(1 for typ in m.nd_types if typ == BIF_nd)}
+    bands = ([q.dmin1[v] for v in range(M)], [q.dmax1[v] for v in range(M)],
+             [q.dmin2[v] for v in range(M)], [q.dmax2[v] for v in range(M)])
+    return W.value, bands
+
+
+def build_cm_from_alignment(ax, ss_cons, name, prior_path, el_self=0.94):
+    """A CM (INFERNAL1/a text) from a digital alignment and its consensus
+    structure with cmbuild's default options, and a dict of summary numbers.
+    Not covered: --hand (RF-defined consensus), truncated sequences,
+    alternative weighting or prior options."""
+    pri = read_prior(prior_path)
+    m, t, e, neff, etarget = estimate(ax, ss_cons, pri)
+    nseq = np.asarray(ax).shape[0]
+    zero = [0] * m.M
+    full = [4 * m.clen] * m.M
+    text0 = cm_text(m, t, e, name, 4 * m.clen, (zero, full, zero, full), nseq, neff, el_self)
+    W, bands = _bands_and_W(text0, m.M)
+    text = cm_text(m, t, e, name, W, bands, nseq, neff, el_self)
+    info = {"clen": m.clen, "M": m.M, "nodes": len(m.nd_types), "W": W, "neff": neff, "etarget": etarget,
+            "nbps": sum(1 for typ in m.nd_types if typ == MATP_nd),
+            "nbif": sum(1 for typ in m.nd_types if typ == BIF_nd)}
     return text, info
+
+
+def build_cm(ss, aln, name, prior_path, el_self=0.94):
+    """A CM from a consensus structure and an ungapped alignment of canonical
+    residues (codes 0..3, one column per consensus position)."""
+    aln = np.asarray(aln, dtype=np.int64)
+    if aln.shape[1] != len(ss):
+        raise ValueError("alignment has %d columns, structure %d" % (aln.shape[1], len(ss)))
+    return build_cm_from_alignment(aln, ss, name, prior_path, el_self)
 
 
 # ---------------------------------------------------------------------------

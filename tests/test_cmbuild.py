@@ -6,9 +6,20 @@ consensus structure of each shipped model is read back from its node lines
 the guide tree, state layout and CMRebalance numbering built from it must
 reproduce the file's node types and every state's type, cfirst, cnum, plast
 and pnum.
+
+Parameters are pinned against the three tutorial models that the current
+cmbuild wrote with default options from alignments the reference ships
+(tutorial/tRNA5.sto, 5S_rRNA.sto, Cobalamin.sto -> the committed
+tRNA5.c.cm, 5S_rRNA.c.cm, Cobalamin.c.cm): the rebuilt model must have the
+same CLEN, STATES, W and EFFN, every transition and emission score within
+the files' print rounding (3 decimals), and the same bands.  The alignments
+and the default prior file are read from the reference tree, so these tests
+skip where it is absent.  (The testsuite models were written by a 2012
+development build with other weighting defaults and are not used here.)
 """
 import os
 
+import numpy as np
 import pytest
 
 from infernal_amd import cmbuild as cb
@@ -66,3 +77,69 @@ def test_topology_matches_shipped_models(name):
 def test_ss_to_ct_wuss():
     ct = cb.ss_to_ct("<<..[[..]]..>>")
     assert ct[1] == 14 and ct[2] == 13 and ct[5] == 10 and ct[6] == 9 and ct[3] == 0
+
+
+REF = os.environ.get("INFERNAL_REFERENCE", "/root/reference")
+PRIOR = os.path.join(REF, "src", "default.prior")
+need_ref = pytest.mark.skipif(not os.path.exists(PRIOR), reason="reference tree (alignments, default prior) not present")
+
+
+def _parse_cm(text):
+    hdr, states = {}, []
+    in_cm = False
+    for line in text.splitlines():
+        f = line.split()
+        if not f:
+            continue
+        if not in_cm:
+            if f[0] == "CM":
+                in_cm = True
+            else:
+                hdr.setdefault(f[0], f[1:])
+            continue
+        if f[0] == "//":
+            break
+        if f[0] != "[":
+            states.append(f)
+    return hdr, states
+
+
+@need_ref
+@pytest.mark.parametrize("name", ["tRNA5", "5S_rRNA", "Cobalamin"])
+def test_build_reproduces_tutorial_models(name):
+    aseqs, gc = cb.read_stockholm(os.path.join(REF, "tutorial", name + ".sto"))
+    ax = cb.digitize_alignment(aseqs)
+    text, info = cb.build_cm_from_alignment(ax, gc["SS_cons"], name, PRIOR)
+    h1, s1 = _parse_cm(text)
+    h2, s2 = _parse_cm(open(os.path.join(CMDIR, name + ".c.cm")).read())
+    for key in ("CLEN", "STATES", "NODES", "W", "NSEQ"):
+        assert h1[key] == h2[key], key
+    assert "%.6f" % float(h1["EFFN"][0]) == h2["EFFN"][0]
+    assert len(s1) == len(s2)
+    band_rows = 0
+    for a, b in zip(s1, s2):
+        assert a[:6] == b[:6]
+        ba, bb = [int(x) for x in a[6:10]], [int(x) for x in b[6:10]]
+        band_rows += ba != bb
+        assert max(abs(x - y) for x, y in zip(ba, bb)) <= 1
+        va = [-1e30 if x == "*" else float(x) for x in a[10:]]
+        vb = [-1e30 if x == "*" else float(x) for x in b[10:]]
+        assert len(va) == len(vb)
+        assert all(abs(x - y) <= 0.0006 for x, y in zip(va, vb)), (a[:2], a[10:], b[10:])
+    assert band_rows <= 1
+
+
+@need_ref
+def test_pb_weights_and_match_columns():
+    """Weights sum to nseq; an all-identical alignment weighs sequences
+    equally; a gap-majority column is an insert column."""
+    ax = cb.digitize_alignment(["ACGU-A", "ACGU-A", "ACGUCA"])
+    w = cb.pb_weights(ax)
+    assert abs(w.sum() - 3) < 1e-12 and np.allclose(w, 1.0)
+    mat = cb.match_columns(ax, w)
+    assert mat[1:7] == [True, True, True, True, False, True]
+    ax = cb.digitize_alignment(["AAAA", "AAAC", "CCCC"])
+    w = cb.pb_weights(ax)
+    # column 4: A once, C twice; columns 1-3: A twice, C once
+    raw = np.array([3 * 1 / 4 + 1 / 2, 3 * 1 / 4 + 1 / 4, 3 * 1 / 2 + 1 / 4]) / 4
+    assert np.allclose(w, raw * 3 / raw.sum())
