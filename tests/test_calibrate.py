@@ -1,0 +1,104 @@
+"""E-values and exponential-tail calibration (infernal_amd.calibrate).
+
+Pinned against the reference's files and against closed forms:
+* every calibrated model shipped with the reference satisfies
+  mu_extrap = mu_orig - ln(1/tailp)/lambda (SetExpInfo) and
+  tailp = (750 local | 250 glocal) x 1.6 Mb / nrandhits (fit_histogram);
+* the E-values of all 56 hits of the tutorial's cmsearch run
+  (tRNA5.c.cm vs mrum-genome.fa, 5,874,406 residues searched) follow from
+  their scores, the model's ECMLI line and UpdateExpsForDBSize, within the
+  output's rounding (scores to 0.1 bit, E-values to 2 digits);
+* the tail fit recovers a planted exponential; score_for_evalue inverts
+  evalue; the generative HMM's samples have its stationary composition.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from infernal_amd import calibrate as cal
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CMDIR = os.path.join(GOLD, "cm")
+CALIBRATED = [f for f in sorted(os.listdir(CMDIR)) if f.endswith(".c.cm")]
+
+
+@pytest.mark.parametrize("name", CALIBRATED)
+def test_shipped_exp_lines_closed_form(name):
+    infos = cal.read_exp_infos(os.path.join(CMDIR, name))
+    assert set(infos) == {"LC", "GC", "LI", "GI"}
+    for mode, e in infos.items():
+        mux = e.mu_orig - math.log(1.0 / e.tailp) / e.lambda_
+        assert abs(mux - e.mu_extrap) < 2e-3, (mode, mux, e.mu_extrap)
+        want_tp = cal.TAILN[mode[0]] * (e.dbsize / 1e6) / e.nrandhits
+        assert abs(want_tp - e.tailp) <= 6e-7 + 1e-4 * want_tp, (mode, want_tp, e.tailp)
+
+
+def _tutorial_hits():
+    rows, z = [], None
+    with open(os.path.join(GOLD, "tRNA5_mrum_hits.tsv")) as fh:
+        for line in fh:
+            if line.startswith("# cmsearch"):
+                z = int(line.rsplit(None, 1)[1])
+            elif not line.startswith("#"):
+                f = line.split("\t")
+                rows.append((float(f[1]), float(f[2])))
+    return rows, z
+
+
+def test_tutorial_evalues():
+    rows, Z = _tutorial_hits()
+    assert len(rows) == 56 and Z == 5874406
+    e = cal.read_exp_infos(os.path.join(CMDIR, "tRNA5.c.cm"))["LI"].update_for_dbsize(Z)
+    tol = e.lambda_ * 0.05 + math.log(1.05) + 1e-6
+    for E, sc in rows:
+        assert abs(math.log(float(e.evalue(sc)) / E)) <= tol, (E, sc, float(e.evalue(sc)))
+    # a plausible slip (one strand only) fails
+    e1 = cal.read_exp_infos(os.path.join(CMDIR, "tRNA5.c.cm"))["LI"].update_for_dbsize(Z / 2)
+    assert any(abs(math.log(float(e1.evalue(sc)) / E)) > tol for E, sc in rows)
+
+
+def test_score_for_evalue_inverts():
+    e = cal.ExpInfo.from_fit(0.7, 3.0, 1.6e6, 250000, 0.004).update_for_dbsize(1e8)
+    for E in (1e-10, 0.01, 1.0, 10.0):
+        sc = e.score_for_evalue(E)
+        assert abs(float(e.evalue(sc)) / E - 1) < 1e-9
+    assert float(e.pvalue(e.mu_extrap - 5)) == 1.0
+
+
+def test_fit_recovers_planted_exponential():
+    rng = np.random.default_rng(3)
+    lam, mu = 0.8, 2.0
+    bulk = rng.normal(-5, 2, 300000)
+    tail = mu + rng.exponential(1 / lam, 3000)
+    x = np.concatenate([bulk, tail])
+    tp = 2000 / len(x)
+    m, l, n = cal.fit_exp_tail(x, tp)
+    assert n == int(len(x) * tp)
+    assert abs(l - lam) / lam < 0.08
+    # the fit's mu is the smallest value kept
+    assert m == np.sort(x)[len(x) - n]
+
+
+def test_tail_mass_rule():
+    assert abs(cal.tail_mass("LC", 1.6e6, 300199) - 0.003997) < 1e-6
+    assert abs(cal.tail_mass("GC", 1.6e6, 77177) - 0.005183) < 1e-6
+    with pytest.raises(ValueError):
+        cal.tail_mass("LI", 1.6e6, 1000)
+
+
+def test_genomic_sampler_composition():
+    rng = np.random.default_rng(11)
+    x = cal.sample_genomic(200, 10000, rng)
+    T = np.asarray(cal._GHMM_TRANS)
+    T = T / T.sum(1, keepdims=True)
+    w, V = np.linalg.eig(T.T)
+    pi = np.real(V[:, np.argmin(abs(w - 1))])
+    pi /= pi.sum()
+    E = np.asarray(cal._GHMM_EMIT)
+    E = E / E.sum(1, keepdims=True)
+    comp = pi @ E
+    got = np.bincount(x.ravel(), minlength=4) / x.size
+    # the chain is sticky (runs of ~thousands), so sampling error is large-ish
+    assert np.abs(got - comp).max() < 0.03, (got, comp)
