@@ -102,3 +102,36 @@ def test_genomic_sampler_composition():
     got = np.bincount(x.ravel(), minlength=4) / x.size
     # the chain is sticky (runs of ~thousands), so sampling error is large-ish
     assert np.abs(got - comp).max() < 0.03, (got, comp)
+
+
+def test_cyk_filter_window_merge(monkeypatch):
+    """pli_cyk_seq_filter's windows: stop-(W-1) .. start+(W-1), clipped,
+    hits in position order, overlapping windows merged."""
+    from infernal_amd import pipeline as pl
+
+    class FakeCM:
+        W = 10
+
+    hits = [(50, 60, 0, 0, 5.0), (5, 12, 0, 0, 9.0), (64, 70, 0, 0, 4.0), (200, 210, 0, 0, 7.0)]
+    seen = {}
+
+    def fake_scan(cm, q, dsq, i0, j0, cutoff, do_null3=True, want_hits=True):
+        seen["args"] = (q, i0, j0, cutoff)
+        return {"hits": hits}
+
+    monkeypatch.setattr(pl, "FastCYKScan", fake_scan)
+    e = cal.ExpInfo.from_fit(0.7, 3.0, 1.6e6, 250000, 0.004).update_for_dbsize(1e6)
+    wins = pl.cyk_filter_windows(FakeCM(), None, 205, e, F6=1e-4)
+    # (5,12)->[3,14]; (50,60)->[51,59]; (64,70)->[61,73] does not touch [51,59]; (200,210)->[201,205]
+    assert wins == [(3, 14), (51, 59), (61, 73), (201, 205)]
+    q, i0, j0, cutoff = seen["args"]
+    assert (q, i0, j0) == (1, 1, 205)
+    assert abs(float(e.pvalue(cutoff)) - 1e-4) < 1e-12
+    hits[:] = [(10, 20, 0, 0, 1.0), (15, 25, 0, 0, 1.0)]
+    # merged: the window ends at the later hit's start + (W-1)
+    assert pl.cyk_filter_windows(FakeCM(), None, 100, e) == [(11, 24)]
+
+
+def test_revcomp():
+    from infernal_amd import pipeline as pl
+    assert pl.revcomp(np.array([0, 1, 2, 3, 3], dtype=np.uint8)).tolist() == [0, 0, 1, 2, 3]
