@@ -525,11 +525,17 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
       return infx_fail(status, errbuf, "%s: device upload failed: %s", fname, hipGetErrorString(g_last_hip));
     P->uploaded = true;
   }
+  /* device tables carry INFX_LOGSUM_PAD entries, the ones past the
+   * reference's 23000 zero */
   if (semi == SEMI_FINSIDE && D->d_lsum_f.cap == 0) {
-    if ((status = upload(D->d_lsum_f, infx_flogsum_table(), LOGSUM_TBL * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+    std::vector<float> t(INFX_LOGSUM_PAD, 0.f);
+    memcpy(t.data(), infx_flogsum_table(), LOGSUM_TBL * 4);
+    if ((status = upload(D->d_lsum_f, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
   }
   if (semi == SEMI_IINSIDE && D->d_lsum_i.cap == 0) {
-    if ((status = upload(D->d_lsum_i, infx_ilogsum_table(), LOGSUM_TBL * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+    std::vector<int> t(INFX_LOGSUM_PAD, 0);
+    memcpy(t.data(), infx_ilogsum_table(), LOGSUM_TBL * 4);
+    if ((status = upload(D->d_lsum_i, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
   }
 
   /* launch shape: wavefronts per workgroup (LDS: log-sum table + per-wave

@@ -31,6 +31,7 @@
 #define EMITNONE  3
 
 #define INFX_LOGSUM_N 23000   /* entries of the FLogsum / ILogsum tables */
+#define INFX_LOGSUM_PAD 23008 /* device copies: entries 23000.. are 0 (FLogsum's 'return max' index) */
 
 enum { KIND_GENERIC = 0, KIND_IL = 1, KIND_B = 2, KIND_BEGL = 3, KIND_SKIP = 4 };
 enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };

@@ -83,6 +83,22 @@ __device__ __forceinline__ void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* Element i of a wave-uniform base pointer, addressed with a 32-bit byte
+ * offset: the load/store takes the SGPR base + VGPR offset form, so no lane
+ * holds a 64-bit address (fewer VGPRs, no 64-bit adds).  Offsets here are
+ * below 2^31 bytes (a slot's rows and decks, the target, the score tables). */
+template <typename T> __device__ __forceinline__ T ldu(const T *base, uint32_t i)
+{ return *(const T *) ((const char *) base + (uint32_t) (i * (uint32_t) sizeof(T))); }
+template <typename T> __device__ __forceinline__ void stu(T *base, uint32_t i, T v)
+{ *(T *) ((char *) base + (uint32_t) (i * (uint32_t) sizeof(T))) = v; }
+
+/* Materialise a loaded value here.  Called on every value of a block right
+ * after all of the block's loads are issued: the loads go out back to back
+ * and are waited for once, and the compiler cannot sink them into the
+ * per-row store branches (which would serialise rows j and j+1 behind the
+ * row-j store). */
+template <typename T> __device__ __forceinline__ void hold(T &x) { asm volatile("" : "+v"(x)); }
+
 __device__ __forceinline__ int wrap_add(int a, int b) { return (int) ((unsigned) a + (unsigned) b); }
 __device__ __forceinline__ int wrap_sub(int a, int b) { return (int) ((unsigned) a - (unsigned) b); }
 __device__ __forceinline__ int wrap_mul(int a, int b) { return (int) ((unsigned) a * (unsigned) b); }
@@ -118,13 +134,16 @@ template <> struct SR<SEMI_FINSIDE> {
   static __device__ __forceinline__ T imp() { return IMPF; }
   static __device__ __forceinline__ T neg_sentinel() { return -INFINITY; }
   /* src/logsum.c:160 FLogsum */
+  /* (max-min) >= 23 or min == -inf returns max: such a difference clamps to
+   * index 23000, whose entry is 0 (max + 0 == max).  A difference below 23
+   * gives (int)(diff*1000) <= 22999 (the float product of the largest float
+   * below 23 and 1000 rounds below 23000).  -inf - -inf is NaN, and
+   * fminf(NaN, 23000) is 23000. */
   static __device__ __forceinline__ T op(T a, T b, const T *tbl) {
     const float mx = fmaxf(a, b), mn = fminf(a, b);
     const float diff = __fsub_rn(mx, mn);
-    const bool  big  = !(diff < 23.f);          /* (max-min) >= 23, or min == -inf */
-    const int   idx  = big ? 0 : (int) __fmul_rn(diff, 1000.0f);
-    const float t    = tbl[idx];
-    return big ? mx : __fadd_rn(mx, t);
+    const int   idx  = (int) fminf(__fmul_rn(diff, 1000.0f), 23000.0f);
+    return __fadd_rn(mx, tbl[idx]);
   }
   static __device__ __forceinline__ T add(T a, T b) { return __fadd_rn(a, b); }
   static __device__ __forceinline__ T bits(uint32_t u) { return __uint_as_float(u); }
@@ -230,10 +249,9 @@ template <int S>
 __device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bool on, int d, typename SR<S>::T sc, typename SR<S>::T bsc)
 {
   typedef SR<S> R;
-  if (on) {
-    const typename R::T cand = R::add(sc, bsc);
-    if (cand >= acc[d - rdn]) acc[d - rdn] = cand;
-  }
+  /* LDS max without return (ds_max_f32 / ds_max_i32): no LDS read to wait
+   * for; lanes hold distinct d, and no value is NaN or -0.0 */
+  if (on) __hip_atomic_fetch_max(&acc[d - rdn], R::add(sc, bsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 /* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
@@ -268,14 +286,17 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
     const int dy0 = (a0 ? d : dn) - sd, dy1 = (a1 ? d : dn) - sd;   /* idle lanes read a valid cell */
     /* left residue j-d+1 (idle lanes: residue j, inside the target even where
      * the row's band is still empty at the start of a chunk) */
-    const int xl0 = (int) dsq_b0[(uint32_t) (dsq_w - (a0 ? d : 1))];
-    const int xl1 = (int) dsq_b1[(uint32_t) (dsq_w - (a1 ? d : 1))];
+    int xl0 = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0 ? d : 1)));
+    int xl1 = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1 ? d : 1)));
     T c0[CN], c1[CN];
 #pragma unroll
     for (int k = 0; k < CN; k++) {
-      c0[k] = crow0[(uint32_t) (cb[k] + dy0)];
-      c1[k] = crow1[(uint32_t) (cb[k] + dy1)];
+      c0[k] = ldu(crow0, (uint32_t) (cb[k] + dy0));
+      c1[k] = ldu(crow1, (uint32_t) (cb[k] + dy1));
     }
+    hold(xl0); hold(xl1);
+#pragma unroll
+    for (int k = 0; k < CN; k++) { hold(c0[k]); hold(c1[k]); }
 #pragma unroll
     for (int k = 0; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
     const T ini0 = R::init(init_ok, el, s.endsc, dy0);
@@ -293,12 +314,12 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
       if (em != EMITNONE) { sc0 = R::add(sc0, e0); sc1 = R::add(sc1, e1); }
     }
     if (a0) {
-      out0[(uint32_t) (s.base + d)] = sc0;
+      stu(out0, (uint32_t) (s.base + d), sc0);
       begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, sc0, bsc);
       if (VSC && vs0) vmax = fmaxf(vmax, R::score(sc0));
     }
     if (a1) {
-      out1[(uint32_t) (s.base + d)] = sc1;
+      stu(out1, (uint32_t) (s.base + d), sc1);
       begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, sc1, bsc);
       if (VSC && vs1) vmax = fmaxf(vmax, R::score(sc1));
     }
@@ -324,23 +345,26 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
   const uint8_t *dsq_b1 = dsq_b0 + 1;
-  T carry0 = cur0[(uint32_t) (s.base + dn - 1)];
-  T carry1 = cur1[(uint32_t) (s.base + dn - 1)];
+  T carry0 = ldu(cur0, (uint32_t) (s.base + dn - 1));
+  T carry1 = ldu(cur1, (uint32_t) (s.base + dn - 1));
   const int dxm = max(dx0, dx1);
   float vmax = IMPF;
   for (int d0 = dn; d0 <= dxm; d0 += 64) {
     const int d = d0 + lane;
     const bool a0 = d <= dx0, a1 = d <= dx1;
     const int dd0 = a0 ? d : max(dn, dx0), dd1 = a1 ? d : max(dn, dx1);   /* idle lanes mirror a valid cell */
-    const int xl0 = (int) dsq_b0[(uint32_t) (dsq_w - (a0 ? d : 1))];   /* idle lanes: residue j */
-    const int xl1 = (int) dsq_b1[(uint32_t) (dsq_w - (a1 ? d : 1))];
+    int xl0 = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0 ? d : 1)));   /* idle lanes: residue j */
+    int xl1 = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1 ? d : 1)));
     T c0[CN], c1[CN];
     c0[0] = R::imp(); c1[0] = R::imp();
 #pragma unroll
     for (int k = 1; k < CN; k++) {
-      c0[k] = cur0[(uint32_t) (cb[k] + dd0 - 1)];
-      c1[k] = cur1[(uint32_t) (cb[k] + dd1 - 1)];
+      c0[k] = ldu(cur0, (uint32_t) (cb[k] + dd0 - 1));
+      c1[k] = ldu(cur1, (uint32_t) (cb[k] + dd1 - 1));
     }
+    hold(xl0); hold(xl1);
+#pragma unroll
+    for (int k = 1; k < CN; k++) { hold(c0[k]); hold(c1[k]); }
 #pragma unroll
     for (int k = 1; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
     const T pre0 = combine_children<S, CN>(c0, R::init(init_ok, el, s.endsc, dd0 - 1), false, tbl);
@@ -360,12 +384,12 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
       if (!__any(changed)) break;
     }
     if (a0) {
-      cur0[(uint32_t) (s.base + d)] = x0;
+      stu(cur0, (uint32_t) (s.base + d), x0);
       begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, x0, bsc);
       if (VSC && vs0) vmax = fmaxf(vmax, R::score(x0));
     }
     if (a1) {
-      cur1[(uint32_t) (s.base + d)] = x1;
+      stu(cur1, (uint32_t) (s.base + d), x1);
       begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, x1, bsc);
       if (VSC && vs1) vmax = fmaxf(vmax, R::score(x1));
     }
@@ -373,6 +397,33 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
     carry1 = R::fromb(__builtin_amdgcn_readlane(R::tob(x1), 63));
   }
   return vmax;
+}
+
+/* Four k of a bifurcation cell: BEGL deck cell (j-k, d-k) + BEGR cell (j, k),
+ * combined into sc in ascending k.  SEL: only k <= lim inside the lane's
+ * own kmin..kmax count. */
+template <int S, bool SEL>
+__device__ __forceinline__ typename SR<S>::T b_step(const typename SR<S>::T *tbl, typename SR<S>::T sc, int yb,
+                                                   const typename SR<S>::T *wbase, int jp, int rmask, int rstride,
+                                                   int kb, int k, int lim, int dd, int kmin, int kmax)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
+  T bl[4], yv[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int kk = k + u;
+    yv[u] = R::fromb(__builtin_amdgcn_readlane(yb, min(kk - kb, 63)));
+    const T *rowp = wbase + (((jp - kk) & rmask) * rstride - kk);
+    bl[u] = ldu(rowp, (uint32_t) dd);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int kk = k + u;
+    const T nsc = R::op(sc, R::add(bl[u], yv[u]), tbl);
+    sc = SEL ? ((kk <= lim && kk >= kmin && kk <= kmax) ? nsc : sc) : nsc;
+  }
+  return sc;
 }
 
 /* B state, one row (src/cm_dpsearch.c:246-284): all lanes walk the union of
@@ -410,30 +461,27 @@ __device__ __forceinline__ float b_state(const typename SR<S>::T *tbl, uint32_t 
       K1 = min(kdx_y, dl - kdn_w);
     } else { kmin = 0; kmax = dd; K0 = 0; K1 = dl; }
     T sc = R::init(init_ok, el, s.endsc, dd);
+    /* k in [kin0, kin1] is inside every lane's own range (kmin grows and
+     * kmax grows with d): those k need no per-lane test */
+    const int kin0 = banded ? max(0, max(kdn_y, dl - kdx_w)) : 0;
+    const int kin1 = banded ? min(kdx_y, d0 - kdn_w) : d0;
     for (int kb = K0; kb <= K1; kb += 64) {
       const int ke = min(K1, kb + 63);
-      const int yb = R::tob(cur[(uint32_t) (by + min(kb + lane, ke))]);
-      /* four k per step, loads first; k > ke is masked (its reads stay
-       * inside the slot's guard cells) */
-      for (int k = kb; k <= ke; k += 4) {
-        T bl[4], yv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int kk = k + u;
-          yv[u] = R::fromb(__builtin_amdgcn_readlane(yb, min(kk - kb, 63)));
-          const T *rowp = wbase + (((jp - kk) & rmask) * rstride - kk);
-          bl[u] = rowp[(uint32_t) dd];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int kk = k + u;
-          const T nsc = R::op(sc, R::add(bl[u], yv[u]), tbl);
-          sc = (kk <= ke && kk >= kmin && kk <= kmax) ? nsc : sc;
-        }
+      const int yb = R::tob(ldu(cur, (uint32_t) (by + min(kb + lane, ke))));
+      /* four k per step, loads first; k past the step's limit is masked
+       * (its reads stay inside the slot's guard cells).  The three loops
+       * visit kb..ke in ascending order, as each lane's sum requires. */
+      const int m0 = max(kb, kin0), m1 = min(ke, kin1);
+      int k = kb;
+      if (m0 <= m1) {
+        for (; k < m0; k += 4) sc = b_step<S, true>(tbl, sc, yb, wbase, jp, rmask, rstride, kb, k, m0 - 1, dd, kmin, kmax);
+        k = m0;
+        for (; k + 3 <= m1; k += 4) sc = b_step<S, false>(tbl, sc, yb, wbase, jp, rmask, rstride, kb, k, m1, dd, kmin, kmax);
       }
+      for (; k <= ke; k += 4) sc = b_step<S, true>(tbl, sc, yb, wbase, jp, rmask, rstride, kb, k, ke, dd, kmin, kmax);
     }
     if (act) {
-      cur[(uint32_t) (s.base + d)] = sc;
+      stu(cur, (uint32_t) (s.base + d), sc);
       begin_update<S>(acc, rdn, hb && d >= bdn && d <= bdx, d, sc, bsc);
       if (VSC && vs) vmax = fmaxf(vmax, R::score(sc));
     }
@@ -455,7 +503,7 @@ __device__ __forceinline__ int begin_argmax(const DState *stp, int nst, const ty
     if (!((s.hdr >> 11) & 1)) continue;
     const int dn = s.dnx & 0xffff, dx = min(jp, (int) (s.dnx >> 16));
     if (d < dn || d > dx) continue;
-    const typename R::T cand = R::add(cur[(uint32_t) (s.base + d)], R::bits(s.beginsc));
+    const typename R::T cand = R::add(ldu(cur, (uint32_t) (s.base + d)), R::bits(s.beginsc));
     if (cand >= bestv) { bestv = cand; br = (int) (s.hdr >> 16); }
   }
   return br;
@@ -479,10 +527,10 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   if (S != SEMI_CYK && TBL_LDS) {
     T *lt = (T *) lds_raw;
     const T *gt = (const T *) fresh(ap)->lsum;
-    for (int i = threadIdx.x; i < LOGSUM_N; i += blockDim.x) lt[i] = gt[i];
+    for (int i = threadIdx.x; i < INFX_LOGSUM_PAD; i += blockDim.x) lt[i] = gt[i];
     __syncthreads();
     tbl = lt;
-    lds_off = LOGSUM_N * 4;
+    lds_off = INFX_LOGSUM_PAD * 4;
   }
   if (slot >= fresh(ap)->nslots) return;
 
@@ -559,8 +607,8 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         int ev0 = 0, ev1 = 0;
         if (em != EMITNONE) {
           const int yy = lane < KP ? lane : 0;
-          ev0 = R::tob(esc[s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy)]);
-          ev1 = (em == EMITPAIR) ? R::tob(esc[s.aux + yy * KP + xj1]) : ev0;
+          ev0 = R::tob(ldu(esc, (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy))));
+          ev1 = (em == EMITPAIR) ? R::tob(ldu(esc, (uint32_t) (s.aux + yy * KP + xj1))) : ev0;
         }
         const bool rd = (hdr >> 7) & 1;
         const T *crow0 = rd ? prv0 : cur0;
@@ -648,9 +696,9 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           float bs = IMPF;
           bool lb = false;
           if (act) {
-            T al = R::clamp_root(R::add(cur[(uint32_t) (aq->root_child_base[0] + d)], R::bits(aq->root_tsc[0])));
+            T al = R::clamp_root(R::add(ldu(cur, (uint32_t) (aq->root_child_base[0] + d)), R::bits(aq->root_tsc[0])));
             for (int k = 1; k < root_cnum; k++)
-              al = R::op(al, R::add(cur[(uint32_t) (aq->root_child_base[k] + d)], R::bits(aq->root_tsc[k])), tbl);
+              al = R::op(al, R::add(ldu(cur, (uint32_t) (aq->root_child_base[k] + d)), R::bits(aq->root_tsc[k])), tbl);
             if (local_begin) {
               const T acv = acc[d - rdn];
               if (acv > al) { al = acv; lb = true; }
@@ -744,7 +792,7 @@ static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds)
 /* LDS bytes of a workgroup of `wpb` wavefronts */
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n)
 {
-  return (semi != SEMI_CYK && tbl_lds ? (size_t) LOGSUM_N * 4 : 0) + (size_t) wpb * acc_n * 8;   /* two rows of 4-byte maxima */
+  return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) + (size_t) wpb * acc_n * 8;   /* two rows of 4-byte maxima */
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
