@@ -254,16 +254,16 @@ __device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bo
   if (on) __hip_atomic_fetch_max(&acc[d - rdn], R::add(sc, bsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
-/* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
- * [dn, dx0] and [dn, dx1] (dx1 < dn: no row 1).  BEGL decks combine init, c0,
- * c1, ... in child order (src/cm_dpsearch.c:430-444); other states use
- * combine_children.  Returns the max score over the rows flagged in vs. */
-template <int S, bool VSC, int CN, bool BEGL>
-__device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
+/* NB consecutive 64-cell blocks of a non-B, non-IL state, rows j (0) and
+ * j+1 (1), from d0: all loads first (2*NB independent cells per lane), then
+ * the combines, then the stores.  Lanes past a row's band compute a copy of
+ * a valid cell and store nothing. */
+template <int S, bool VSC, int CN, bool BEGL, int NB>
+__device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
                                             const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
-                                            typename SR<S>::T *out0, typename SR<S>::T *out1,
-                                            int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                            int ev0, int ev1, bool vs0, bool vs1)
+                                            typename SR<S>::T *out0, typename SR<S>::T *out1, const int *cb,
+                                            int d0, int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
+                                            int ev0, int ev1, typename SR<S>::T er0, typename SR<S>::T er1, bool vs0, bool vs1)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -272,58 +272,98 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
   const int init_ok = (hdr >> 12) & 1;
   const T bsc = R::bits(s.beginsc);
   const int bdn = max(rdn, dn);
+  const uint8_t *dsq_b1 = dsq_b0 + 1;
+  float vmax = IMPF;
+  int d[NB], dy0[NB], dy1[NB], xl0[NB], xl1[NB];
+  bool a0[NB], a1[NB];
+  T c0[NB][CN], c1[NB][CN];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    d[b] = d0 + 64 * b + lane;
+    a0[b] = d[b] <= dx0; a1[b] = d[b] <= dx1;
+    dy0[b] = (a0[b] ? d[b] : dn) - sd; dy1[b] = (a1[b] ? d[b] : dn) - sd;   /* idle lanes read a valid cell */
+    /* left residue j-d+1 (idle lanes: residue j, inside the target even where
+     * the row's band is still empty at the start of a chunk) */
+    xl0[b] = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0[b] ? d[b] : 1)));
+    xl1[b] = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1[b] ? d[b] : 1)));
+#pragma unroll
+    for (int k = 0; k < CN; k++) {
+      c0[b][k] = ldu(crow0, (uint32_t) (cb[k] + dy0[b]));
+      c1[b][k] = ldu(crow1, (uint32_t) (cb[k] + dy1[b]));
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    hold(xl0[b]); hold(xl1[b]);
+#pragma unroll
+    for (int k = 0; k < CN; k++) { hold(c0[b][k]); hold(c1[b][k]); }
+  }
+  T sc0[NB], sc1[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+#pragma unroll
+    for (int k = 0; k < CN; k++) { c0[b][k] = R::add(c0[b][k], R::bits(s.tsc[k])); c1[b][k] = R::add(c1[b][k], R::bits(s.tsc[k])); }
+    const T ini0 = R::init(init_ok, el, s.endsc, dy0[b]);
+    const T ini1 = R::init(init_ok, el, s.endsc, dy1[b]);
+    if (BEGL) {
+      sc0[b] = ini0; sc1[b] = ini1;
+#pragma unroll
+      for (int k = 0; k < CN; k++) { sc0[b] = R::op(sc0[b], c0[b][k], tbl); sc1[b] = R::op(sc1[b], c1[b][k], tbl); }
+    } else {
+      sc0[b] = combine_children<S, CN>(c0[b], ini0, true, tbl);
+      sc1[b] = combine_children<S, CN>(c1[b], ini1, true, tbl);
+      const T e0 = (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0[b]);
+      const T e1 = (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1[b]);
+      if (em != EMITNONE) { sc0[b] = R::add(sc0[b], e0); sc1[b] = R::add(sc1[b], e1); }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    if (a0[b]) {
+      stu(out0, (uint32_t) (s.base + d[b]), sc0[b]);
+      begin_update<S>(acc0, rdn, hb0 && d[b] >= bdn && d[b] <= bdx0, d[b], sc0[b], bsc);
+      if (VSC && vs0) vmax = fmaxf(vmax, R::score(sc0[b]));
+    }
+    if (a1[b]) {
+      stu(out1, (uint32_t) (s.base + d[b]), sc1[b]);
+      begin_update<S>(acc1, rdn, hb1 && d[b] >= bdn && d[b] <= bdx1, d[b], sc1[b], bsc);
+      if (VSC && vs1) vmax = fmaxf(vmax, R::score(sc1[b]));
+    }
+  }
+  return vmax;
+}
+
+/* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
+ * [dn, dx0] and [dn, dx1] (dx1 < dn: no row 1).  BEGL decks combine init, c0,
+ * c1, ... in child order (src/cm_dpsearch.c:430-444); other states use
+ * combine_children.  WIDE (Inside with the LDS table) takes two blocks at a
+ * time where the band has them: its log-sum chains wait on LDS, and it runs
+ * at 4 waves per SIMD, so the registers are free.  Returns the max score over the rows
+ * flagged in vs. */
+template <int S, bool VSC, int CN, bool BEGL, bool WIDE>
+__device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
+                                            const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
+                                            typename SR<S>::T *out0, typename SR<S>::T *out1,
+                                            int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
+                                            int ev0, int ev1, bool vs0, bool vs1)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
   const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
   const T er1 = R::fromb(__builtin_amdgcn_readlane(ev1, xj1));
-  const uint8_t *dsq_b1 = dsq_b0 + 1;
   const int dxm = max(dx0, dx1);
   float vmax = IMPF;
-  for (int d0 = dn; d0 <= dxm; d0 += 64) {
-    const int d = d0 + lane;
-    const bool a0 = d <= dx0, a1 = d <= dx1;
-    const int dy0 = (a0 ? d : dn) - sd, dy1 = (a1 ? d : dn) - sd;   /* idle lanes read a valid cell */
-    /* left residue j-d+1 (idle lanes: residue j, inside the target even where
-     * the row's band is still empty at the start of a chunk) */
-    int xl0 = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0 ? d : 1)));
-    int xl1 = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1 ? d : 1)));
-    T c0[CN], c1[CN];
-#pragma unroll
-    for (int k = 0; k < CN; k++) {
-      c0[k] = ldu(crow0, (uint32_t) (cb[k] + dy0));
-      c1[k] = ldu(crow1, (uint32_t) (cb[k] + dy1));
-    }
-    hold(xl0); hold(xl1);
-#pragma unroll
-    for (int k = 0; k < CN; k++) { hold(c0[k]); hold(c1[k]); }
-#pragma unroll
-    for (int k = 0; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
-    const T ini0 = R::init(init_ok, el, s.endsc, dy0);
-    const T ini1 = R::init(init_ok, el, s.endsc, dy1);
-    T sc0, sc1;
-    if (BEGL) {
-      sc0 = ini0; sc1 = ini1;
-#pragma unroll
-      for (int k = 0; k < CN; k++) { sc0 = R::op(sc0, c0[k], tbl); sc1 = R::op(sc1, c1[k], tbl); }
-    } else {
-      sc0 = combine_children<S, CN>(c0, ini0, true, tbl);
-      sc1 = combine_children<S, CN>(c1, ini1, true, tbl);
-      const T e0 = (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0);
-      const T e1 = (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1);
-      if (em != EMITNONE) { sc0 = R::add(sc0, e0); sc1 = R::add(sc1, e1); }
-    }
-    if (a0) {
-      stu(out0, (uint32_t) (s.base + d), sc0);
-      begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, sc0, bsc);
-      if (VSC && vs0) vmax = fmaxf(vmax, R::score(sc0));
-    }
-    if (a1) {
-      stu(out1, (uint32_t) (s.base + d), sc1);
-      begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, sc1, bsc);
-      if (VSC && vs1) vmax = fmaxf(vmax, R::score(sc1));
-    }
-  }
+  int d0 = dn;
+  if (WIDE)
+    for (; d0 + 64 <= dxm; d0 += 128)
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, er0, er1, vs0, vs1));
+  for (; d0 <= dxm; d0 += 64)
+    vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                       hb0, hb1, bdx0, bdx1, ev0, ev1, er0, er1, vs0, vs1));
   return vmax;
 }
 
@@ -619,7 +659,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           const bool seq = (hdr >> 14) & 1;
           const int gx1 = seq ? dn - 1 : dx1;
           switch (cnum) {
-#define GEN(CN) vmax = gen_state2<S, VSC, CN, false>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
           case 2:  GEN(2); break;
           case 3:  GEN(3); break;
           case 4:  GEN(4); break;
@@ -631,7 +671,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
             wave_sync();
             float vm1;
             switch (cnum) {
-#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
             case 2:  GEN(2); break;
             case 3:  GEN(3); break;
             case 4:  GEN(4); break;
@@ -656,7 +696,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           T *out0 = deck + (jp0 & rmask) * rstride;
           T *out1 = deck + (jp1 & rmask) * rstride;
           switch (cnum) {
-#define BGL(CN) vmax = gen_state2<S, VSC, CN, true>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
+#define BGL(CN) vmax = gen_state2<S, VSC, CN, true, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
           case 1:  BGL(1); break;
           case 2:  BGL(2); break;
           case 3:  BGL(3); break;
