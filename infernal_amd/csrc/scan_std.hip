@@ -263,7 +263,7 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
                                             const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
                                             typename SR<S>::T *out0, typename SR<S>::T *out1, const int *cb,
                                             int d0, int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                            int ev0, int ev1, typename SR<S>::T er0, typename SR<S>::T er1, bool vs0, bool vs1)
+                                            int ev0, int ev1, bool vs0, bool vs1)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -298,6 +298,9 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
 #pragma unroll
     for (int k = 0; k < CN; k++) { hold(c0[b][k]); hold(c1[b][k]); }
   }
+  /* right residue's score (EMITRIGHT): read after the loads are issued */
+  const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
+  const T er1 = R::fromb(__builtin_amdgcn_readlane(ev1, xj1));
   T sc0[NB], sc1[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
@@ -352,18 +355,16 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
-  const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
-  const T er1 = R::fromb(__builtin_amdgcn_readlane(ev1, xj1));
   const int dxm = max(dx0, dx1);
   float vmax = IMPF;
   int d0 = dn;
   if (WIDE)
     for (; d0 + 64 <= dxm; d0 += 128)
       vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, er0, er1, vs0, vs1));
+                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
   for (; d0 <= dxm; d0 += 64)
     vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                       hb0, hb1, bdx0, bdx1, ev0, ev1, er0, er1, vs0, vs1));
+                                                       hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
   return vmax;
 }
 
@@ -644,12 +645,17 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         const int em = (hdr >> 3) & 3;
         const int bdx0 = min(rdx0, dx0), bdx1 = min(rdx1, dx1);
         /* emission rows: lane y <- score of residue code y (MP: left code y, right code xj) */
-        int ev0 = 0, ev1 = 0;
-        if (em != EMITNONE) {
-          const int yy = lane < KP ? lane : 0;
-          ev0 = R::tob(ldu(esc, (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy))));
-          ev1 = (em == EMITPAIR) ? R::tob(ldu(esc, (uint32_t) (s.aux + yy * KP + xj1))) : ev0;
-        }
+        /* Loaded for every state (non-emitting states read entry 0 and never
+         * use it): a branch here would join a loaded and a non-loaded value,
+         * and the register write on the other path has to wait for any load
+         * still in flight -- the previous state's stores included.  Two
+         * independent loads (the same row twice for singlets), so no value is
+         * needed before the state's child loads are issued. */
+        const int yy = lane < KP ? lane : 0;
+        const uint32_t eo0 = (em == EMITNONE) ? 0u : (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy));
+        const uint32_t eo1 = (em == EMITNONE) ? 0u : (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj1 : yy));
+        const int ev0 = R::tob(ldu(esc, eo0));
+        const int ev1 = R::tob(ldu(esc, eo1));
         const bool rd = (hdr >> 7) & 1;
         const T *crow0 = rd ? prv0 : cur0;
         const T *crow1 = rd ? cur0 : cur1;
