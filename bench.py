@@ -90,7 +90,7 @@ def cyk_1500(args, ia, synth, threads):
     frags = synth.homologs(ia.CM.read(path), max(1, n // 20000), seed=15)
     codes, _ = synth.plant(bg, frags, seed=16)
     dsq = ia.dsq_from_codes(codes)
-    ia.FastCYKScan(cm, 1, dsq, 1, min(n, 200000), 0.0, want_hits=True)
+    ia.FastCYKScan(cm, 1, dsq, 1, min(n, 1600), 0.0, want_hits=True)   # first-use set-up, one short chunk
     t0 = time.perf_counter()
     r = ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
     wall = time.perf_counter() - t0
@@ -122,6 +122,83 @@ def cyk_1500(args, ia, synth, threads):
     return out
 
 
+def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
+    """The learned gate on this workload: trained on a separate seeded
+    target (10% emitted homologs) against the plain Inside's own hits, then
+    run on the benchmark target in place of the plain CYK + Inside pass.
+    Plain-method hits: FastFInsideScan hits with E <= 0.01 (the model's
+    local-Inside tail, one strand of the target).  Returns the JSON record."""
+    import torch
+    from infernal_amd import gate as gt
+    from infernal_amd.calibrate import read_exp_infos
+    exps = read_exp_infos(cm_path)
+    if "LI" not in exps:
+        return {"skipped": "model has no E-value calibration"}
+    n = len(codes)
+    li = exps["LI"].update_for_dbsize(float(n))
+    T = li.score_for_evalue(0.01)
+    cm = ia.CM.read(cm_path)
+    cm.configure(local=True)
+    W = cm.W
+    # training target and its plain labels (Inside hits down to T - 5 bits)
+    t0 = time.perf_counter()
+    ntr = int(args.gate_train_mb * 1e6)
+    bg = synth.markov2_background(ntr, gc=0.40, seed=4242)
+    nh = max(1, int(0.10 * ntr / (0.75 * cm.clen)))
+    frags = synth.homologs(ia.CM.read(cm_path), nh, seed=4243)
+    tcodes, _ = synth.plant(bg, frags, seed=4244)
+    tdsq = ia.dsq_from_codes(tcodes)
+    r = ia.FastFInsideScan(cm, 2, tdsq, 1, ntr, T - 5.0, want_hits=True)
+    lab = gt.labels_from_hits(ntr, [(h[0], h[1]) for h in r["hits"]])
+    train_hits = [(h[0], h[1], h[4]) for h in r["hits"] if h[4] >= T]
+    t_data = time.perf_counter() - t0
+    # training
+    net = gt.GateNet(seed=7)
+    seg, nseg = 1024, 32
+    rng = np.random.default_rng(5)
+    dcodes = torch.from_numpy(np.ascontiguousarray(tcodes)).cuda()
+    dlab = torch.from_numpy(lab).cuda()
+    pos = float(lab.mean())
+    pw = min(20.0, max(1.0, (1 - pos) / max(pos, 1e-6)) ** 0.5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(args.gate_steps):
+        st = rng.integers(0, ntr - seg, nseg)
+        idx = (torch.from_numpy(st)[:, None] + torch.arange(seg)[None, :]).reshape(-1).cuda()
+        losses.append(net.train_step(dcodes[idx].contiguous(), dlab[idx].contiguous(), seg, lr=2e-3, pos_weight=pw))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # threshold: keep >= 99.5% of the training target's plain hits
+    tl = net.score(tcodes)
+    per_hit = np.array([tl[min(a, b) - 1:max(a, b)].max() for a, b, _ in train_hits]) if train_hits else np.zeros(1)
+    tau = float(np.quantile(per_hit, 0.005)) - 0.5
+    # the gated pass on the benchmark target
+    ia.FastFInsideScan(cm, 2, dsq, 1, min(n, 200000), T, want_hits=True)    # warm profile
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    logits = net.score(codes)
+    t_score = time.perf_counter() - t0
+    wins = gt.select_windows(logits, tau, W)
+    t1 = time.perf_counter()
+    ghits = gt.gated_inside(cm, dsq, wins, T)
+    t_inside = time.perf_counter() - t1
+    t_gate = t_score + (t1 - t0 - t_score) + t_inside
+    keep = [h for h in plain_hits if h[2] >= T]
+    rec = gt.recall(keep, ghits)
+    log("gate: train %.1f s (%d steps, loss %.4f -> %.4f), tau %.2f, %d windows (%.2f%% of residues), "
+        "score %.2f s, inside %.2f s, recall %.4f of %d hits" % (t_train, args.gate_steps, losses[0], losses[-1], tau,
+                                                               len(wins), 100.0 * sum(j - i + 1 for i, j in wins) / n,
+                                                               t_score, t_inside, rec, len(keep)))
+    return {"speedup": round(plain_s / t_gate, 2), "recall": round(rec, 5), "plain_hits_e001": len(keep),
+            "score_threshold_bits": round(T, 2), "gate_s": round(t_gate, 3), "plain_s": round(plain_s, 2),
+            "gate_score_s": round(t_score, 3), "gated_inside_s": round(t_inside, 3), "windows": len(wins),
+            "residue_fraction": round(sum(j - i + 1 for i, j in wins) / n, 5), "tau": round(tau, 3),
+            "train": {"target_mb": ntr / 1e6, "homologs": nh, "steps": args.gate_steps, "seconds": round(t_train, 1),
+                      "label_seconds": round(t_data, 1), "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),
+                      "train_hits": len(train_hits)}}
+
+
 def _cells(cm, qdbidx, i0, j0):
     import ctypes as C
     import infernal_amd as ia
@@ -145,7 +222,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8000, help="residues per CPU-baseline window")
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--cyk1500-mb", type=float, default=10.0,
+    ap.add_argument("--gate-steps", type=int, default=400, help="learned-gate training steps (0 = skip the gate leg)")
+    ap.add_argument("--gate-train-mb", type=float, default=8.0, help="learned-gate training target in megabases")
+    ap.add_argument("--cyk1500-mb", type=float, default=20.0,
                     help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -214,6 +293,7 @@ def main():
             tm = ia.last_call_timing()
             kern["finside"] += tm["kernel_ms"]; cells_by["finside"] += tm["cells"]
             hits["finside"] = len(r["hits"])
+            plain_hits = [(h[0], h[1], h[4]) for h in r["hits"]]
             log("step %d FInside: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (s, tm["wall_ms"] / 1e3, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
         step_times.append(time.perf_counter() - t0)
     total_t = sum(step_times)
@@ -252,6 +332,10 @@ def main():
                "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
                "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4)}
 
+    gate = None
+    if args.gate_steps > 0 and args.only == "both":
+        gate = gate_leg(args, ia, synth, args.cm, codes, dsq, plain_hits, total_t / args.steps)
+
     big = None
     if args.cyk1500_mb > 0:
         big = cyk_1500(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
@@ -275,7 +359,8 @@ def main():
                    "cells_per_step": cells_total / args.steps, "hits": hits},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "learned_gate_speedup": None,
+        "learned_gate_speedup": (gate["speedup"] if gate and gate.get("recall", 0) >= 0.99 else None),
+        "learned_gate": gate,
         "cyk_1500nt": big,
     }
     print(json.dumps(out))
