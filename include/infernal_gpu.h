@@ -407,6 +407,33 @@ int infx_CalculateQueryDependentBands(CM_t *cm, char *errbuf, CM_QDBINFO *qdbinf
  * block.  Used by the autotuner and tests. */
 void infx_set_launch_params(int chunk_len, int waves_per_block);
 
+/* ---- learned window gate (no counterpart in the reference) ------------
+ * A per-model 1-D convolutional scorer over one-hot ACGU (4 convolutions of
+ * width 9 and 128 channels with ReLU, then a per-position linear head), in
+ * bf16 on the matrix cores with fp32 accumulation, and its training
+ * kernels.  It picks the stretches of a target that go to the exact Inside
+ * scan; the plain scan remains the fallback and the judge of its misses.
+ * Device pointers, default stream; return 0 or a HIP error code.
+ * Activations are [position][channel]; a batch is equal-length segments of
+ * `seg` positions laid end to end.  Weights: [128][kp] with kp = 64 (first
+ * layer, 4 input channels) or 1152. */
+int infx_gate_onehot(const uint8_t *codes, int64_t n, void *x_bf16);
+/* mode 0: ReLU, bf16 out; 1: bf16 out; 2: fp32 out */
+int infx_gate_conv(const void *x_bf16, int64_t n, int seg, int cin, const void *w_bf16, int kp, const float *bias,
+                   void *y, int mode);
+/* dw[co][kk] += sum_t g[t][co] * x[t+k-4][ci] (fp32 atomics; clear dw first) */
+int infx_gate_wgrad(const void *g_bf16, const void *x_bf16, int64_t n, int seg, int cin, int kp, float *dw);
+int infx_gate_relu_bwd(void *g_bf16, const void *y_bf16, int64_t m);
+/* out[c] += sum_t g[t][c] (clear out first): bias gradients */
+int infx_gate_colsum(const void *g_bf16, int64_t n, float *out);
+int infx_gate_head(const void *y_bf16, int64_t n, const float *hw, const float *hb, float *s);
+int infx_gate_head_bwd(const float *s, const float *labels, const void *y_bf16, int64_t n, const float *hw, float pos_weight,
+                       void *dy_bf16, float *dhw, float *dhb, float *loss);
+int infx_gate_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps, int step);
+/* fp32 master [128][kp] -> bf16 forward copy and (cin 128) the k-flipped,
+ * transposed copy the data gradient runs with */
+int infx_gate_pack(const float *w, int kp, int cin, void *wf_bf16, void *wd_bf16);
+
 #ifdef __cplusplus
 }
 #endif

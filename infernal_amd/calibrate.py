@@ -92,6 +92,25 @@ class ExpInfo:
         return self.mu_extrap + math.log(E / self.cur_eff_dbsize) / (-self.lambda_)
 
 
+def exp_lines(infos):
+    """ECM lines of a CM file header for {mode: ExpInfo} (cm_file.c's order:
+    LC, GC, LI, GI)."""
+    out = []
+    for mode in ("LC", "GC", "LI", "GI"):
+        if mode in infos:
+            e = infos[mode]
+            out.append("ECM%s    %.5f  %10.5f  %10.5f  %10d  %10d  %.6f" % (mode, e.lambda_, e.mu_extrap, e.mu_orig,
+                                                                          int(e.dbsize), e.nrandhits, e.tailp))
+    return out
+
+
+def with_exp_lines(cm_text, infos):
+    """The model text with its ECM lines replaced by those of infos."""
+    lines = [l for l in cm_text.splitlines() if not l.startswith("ECM")]
+    k = next(i for i, l in enumerate(lines) if l.split() and l.split()[0] == "CM")
+    return "\n".join(lines[:k] + exp_lines(infos) + lines[k:]) + "\n"
+
+
 def read_exp_infos(cm_path):
     """ECMLC/ECMGC/ECMLI/ECMGI lines of a calibrated CM file, by mode name."""
     out = {}
@@ -148,14 +167,20 @@ def sample_genomic(nseq, L, rng):
     return out
 
 
-def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk=10000, null3=True):
+def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk=10000, null3=True, joined=False):
     """cmcalibrate's exponential tails for the model in cm_path, the scans
-    on the GPU.  Returns {mode: ExpInfo} and the hit scores per mode."""
+    on the GPU.  Returns {mode: ExpInfo} and the hit scores per mode.
+    joined: scan the generated pieces end to end as one target (one launch
+    per mode instead of one per piece; hits may then span the junctions
+    between pieces, a few in a thousand for W of a few hundred)."""
     from . import CM, FastCYKScan, FastIInsideScan, dsq_from_codes
     nseq = int(L_mb * 1e6 / chunk + 0.5)
     total = nseq * chunk
     rng = np.random.default_rng(seed)
     seqs = sample_genomic(nseq, chunk, rng)
+    if joined:
+        seqs = [seqs.reshape(-1)]
+        chunk = total
     dsqs = [dsq_from_codes(s) for s in seqs]
     res, all_scores = {}, {}
     for mode in modes:

@@ -135,3 +135,17 @@ def test_cyk_filter_window_merge(monkeypatch):
 def test_revcomp():
     from infernal_amd import pipeline as pl
     assert pl.revcomp(np.array([0, 1, 2, 3, 3], dtype=np.uint8)).tolist() == [0, 0, 1, 2, 3]
+
+
+def test_exp_lines_round_trip(tmp_path):
+    src = os.path.join(CMDIR, "tRNA5.c.cm")
+    infos = cal.read_exp_infos(src)
+    text = cal.with_exp_lines(open(src).read(), infos)
+    assert sum(1 for l in text.splitlines() if l.startswith("ECM")) == 4
+    p = tmp_path / "x.cm"
+    p.write_text(text)
+    back = cal.read_exp_infos(str(p))
+    for m in infos:
+        a, b = infos[m], back[m]
+        assert abs(a.lambda_ - b.lambda_) < 1e-5 and abs(a.mu_extrap - b.mu_extrap) < 1e-5
+        assert a.nrandhits == b.nrandhits and abs(a.tailp - b.tailp) < 1e-6
