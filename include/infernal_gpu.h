@@ -14,6 +14,7 @@
 #ifndef INFERNAL_GPU_H
 #define INFERNAL_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -403,6 +404,16 @@ int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int6
 int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W);
 int infx_CalculateQueryDependentBands(CM_t *cm, char *errbuf, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W, int *ret_Z);
 
+/* Many windows wi[w]..wj[w] (sorted, disjoint, 1-based) of one target in one
+ * launch, with FastCYKScan (semi 0), FastFInsideScan (1) or FastIInsideScan
+ * (2) semantics: the hits appended to hitlist and *ret_sc are those of one
+ * standard scan per window.  No counterpart in the reference: for callers
+ * that scan many short windows (the pipeline's final stage, the learned
+ * gate), where one launch per window would leave the GPU idle. */
+int infx_FastScanWindows(int semi, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,
+                         const int64_t *wi, const int64_t *wj, int nwin, float cutoff, CM_TOPHITS *hitlist,
+                         int do_null3, float *ret_sc);
+
 /* Override the work split (0 = automatic): residues per chunk and waves per
  * block.  Used by the autotuner and tests. */
 void infx_set_launch_params(int chunk_len, int waves_per_block);
@@ -411,18 +422,29 @@ void infx_set_launch_params(int chunk_len, int waves_per_block);
  * A per-model 1-D convolutional scorer over one-hot ACGU (4 convolutions of
  * width 9 and 128 channels with ReLU, then a per-position linear head), in
  * bf16 on the matrix cores with fp32 accumulation, and its training
- * kernels.  It picks the stretches of a target that go to the exact Inside
- * scan; the plain scan remains the fallback and the judge of its misses.
+ * kernels (width 9, dilation per layer).  It picks the stretches of a target
+ * that go to the exact Inside scan; the plain scan remains the fallback and
+ * the judge of its misses.
  * Device pointers, default stream; return 0 or a HIP error code.
  * Activations are [position][channel]; a batch is equal-length segments of
  * `seg` positions laid end to end.  Weights: [128][kp] with kp = 64 (first
  * layer, 4 input channels) or 1152. */
+/* device memory from the library's own runtime (synchronous) */
+int infx_dev_malloc(size_t bytes, void **ptr);
+int infx_dev_free(void *ptr);
+int infx_dev_memset(void *ptr, int value, size_t bytes);
+int infx_dev_upload(void *dst, const void *src, size_t bytes);
+int infx_dev_download(void *dst, const void *src, size_t bytes);
+int infx_dev_sync(void);
+/* training batch: nseg segments of seg residues from starts[] (device) */
+int infx_gate_gather(const uint8_t *codes, const float *labels, const int64_t *starts, int nseg, int seg,
+                     uint8_t *out_codes, float *out_labels);
 int infx_gate_onehot(const uint8_t *codes, int64_t n, void *x_bf16);
 /* mode 0: ReLU, bf16 out; 1: bf16 out; 2: fp32 out */
-int infx_gate_conv(const void *x_bf16, int64_t n, int seg, int cin, const void *w_bf16, int kp, const float *bias,
+int infx_gate_conv(const void *x_bf16, int64_t n, int seg, int cin, int dil, const void *w_bf16, int kp, const float *bias,
                    void *y, int mode);
-/* dw[co][kk] += sum_t g[t][co] * x[t+k-4][ci] (fp32 atomics; clear dw first) */
-int infx_gate_wgrad(const void *g_bf16, const void *x_bf16, int64_t n, int seg, int cin, int kp, float *dw);
+/* dw[co][kk] += sum_t g[t][co] * x[t+(k-4)*dil][ci] (fp32 atomics; clear dw first) */
+int infx_gate_wgrad(const void *g_bf16, const void *x_bf16, int64_t n, int seg, int cin, int dil, int kp, float *dw);
 int infx_gate_relu_bwd(void *g_bf16, const void *y_bf16, int64_t m);
 /* out[c] += sum_t g[t][c] (clear out first): bias gradients */
 int infx_gate_colsum(const void *g_bf16, int64_t n, float *out);

@@ -300,6 +300,39 @@ def _scan(fname, cm, qdbidx, dsq, i0, j0, cutoff, do_null3, env_cutoff, want_vsc
     return {"hits": hits, "envi": envi.value, "envj": envj.value, "sc": sc.value, "vsc": v, "mode": mode}
 
 
+def ScanWindows(algo, cm, qdbidx, dsq, windows, cutoff, do_null3=False, greedy=True):
+    """infx_FastScanWindows: many windows [(i, j), ...] (sorted, disjoint) of
+    one target in one launch; algo "cyk", "finside" or "iinside".  The hits
+    are those of one standard scan per window."""
+    semi = {"cyk": 0, "finside": 1, "iinside": 2}[algo]
+    c = cm.c
+    if semi:
+        c.search_opts |= CM_SEARCH_INSIDE
+    else:
+        c.search_opts &= ~CM_SEARCH_INSIDE
+    if greedy:
+        c.search_opts &= ~CM_SEARCH_CMNOTGREEDY
+    else:
+        c.search_opts |= CM_SEARCH_CMNOTGREEDY
+    dsq = np.ascontiguousarray(dsq, dtype=np.uint8)
+    wi = np.ascontiguousarray([w[0] for w in windows], dtype=np.int64)
+    wj = np.ascontiguousarray([w[1] for w in windows], dtype=np.int64)
+    eb = C.create_string_buffer(ERRBUF)
+    th = _lib.cm_tophits_Create()
+    sc = C.c_float()
+    f = _lib.infx_FastScanWindows
+    f.argtypes = [C.c_int, C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                  C.c_float, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    st = f(semi, C.cast(cm.ptr, C.c_void_p), eb, C.cast(c.smx, C.c_void_p), qdbidx, dsq.ctypes.data, wi.ctypes.data,
+           wj.ctypes.data, len(windows), cutoff, th, int(do_null3), C.byref(sc))
+    if st != eslOK:
+        _lib.cm_tophits_Destroy(th)
+        raise ScanError(st, eb.value.decode(errors="replace"))
+    hits = _hits_list(th)
+    _lib.cm_tophits_Destroy(th)
+    return {"hits": hits, "sc": sc.value}
+
+
 def FastCYKScan(cm, qdbidx, dsq, i0, j0, cutoff, do_null3=False, env_cutoff=0.0, want_vsc=False, want_hits=True, greedy=True):
     """src/cm_dpsearch.c:90 FastCYKScan on the GPU."""
     return _scan("FastCYKScan", cm, qdbidx, dsq, i0, j0, cutoff, do_null3, env_cutoff, want_vsc, want_hits, greedy=greedy)

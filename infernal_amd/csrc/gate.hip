@@ -3,13 +3,13 @@
  * its own training kernels.  No counterpart in the reference: it sits in
  * front of the exact Inside scan (which stays the judge of its misses).
  *
- * Network: 4 convolutions (kernel 9, padding 4, 128 channels, ReLU), then a
- * per-position linear head (128 -> 1 logit).  Activations are channels-last
+ * Network: 4 convolutions (kernel 9, dilation dil, padding 4*dil, 128
+ * channels, ReLU), then a per-position linear head (128 -> 1 logit).  Activations are channels-last
  * [position][channel] bf16; a batch is a run of equal-length segments laid
  * end to end (no convolution reads across a segment edge).
  *
  * A convolution is an implicit GEMM: out[t][co] = sum over (k, ci) of
- * in[t+k-4][ci] * w[co][k*Cin+ci].  The reduction axis kk = k*Cin+ci is
+ * in[t+(k-4)*dil][ci] * w[co][k*Cin+ci].  The reduction axis kk = k*Cin+ci is
  * padded to Kp (a multiple of 16) with zero weights.  One wavefront computes
  * a 16-position x 128-channel tile with v_mfma_f32_16x16x16_bf16
  * (A: lane l holds rows l%16, k 4*(l/16)..+3; B: columns l%16, same k;
@@ -56,7 +56,7 @@ __global__ void gate_onehot_kernel(const uint8_t *codes, int64_t n, bf16_t *x)
  * x [n][cin] bf16, w [128][kp] bf16, bias [128] fp32 (may be null),
  * y [n][128] bf16 or yf [n][128] fp32.  cin is 4 or 128. */
 template <bool RELU, bool OUTF32>
-__global__ void __launch_bounds__(256) gate_conv_kernel(const bf16_t *x, int64_t n, int seg, int cin,
+__global__ void __launch_bounds__(256) gate_conv_kernel(const bf16_t *x, int64_t n, int seg, int cin, int dil,
                                                         const bf16_t *w, int kp, const float *bias,
                                                         bf16_t *y, float *yf)
 {
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) gate_conv_kernel(const bf16_t *x, int64_t
     s4 a = (s4){0, 0, 0, 0};
     if (kk < kmax && t < n) {
       const int k = kk / cin, ci = kk - k * cin;
-      const int64_t src = t + k - 4;
+      const int64_t src = t + (int64_t) (k - 4) * dil;
       if (src >= sbeg && src < send) a = *(const s4 *) (x + src * cin + ci);
     }
 #pragma unroll
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) gate_conv_kernel(const bf16_t *x, int64_t
  * grid = (128/16, kp/64, nsplit), block 64; each workgroup sums positions
  * [z*tchunk, (z+1)*tchunk). */
 __global__ void __launch_bounds__(64) gate_wgrad_kernel(const bf16_t *g, const bf16_t *x, int64_t n, int seg, int cin,
-                                                        int kp, int64_t tchunk, float *dw)
+                                                        int dil, int kp, int64_t tchunk, float *dw)
 {
   const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
   const int co0 = blockIdx.x * 16, kb = blockIdx.y * 64;
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(64) gate_wgrad_kernel(const bf16_t *g, const b
         bv[i] = 0;
         if (t < tz1 && kcol[c] >= 0) {
           const int64_t sbeg = (t / seg) * seg, send = min(sbeg + seg, n);
-          const int64_t src = t + kcol[c] - 4;
+          const int64_t src = t + (int64_t) (kcol[c] - 4) * dil;
           if (src >= sbeg && src < send) bv[i] = x[src * cin + cicol[c]];
         }
       }
@@ -250,10 +250,42 @@ __global__ void gate_pack_kernel(const float *w, int kp, int cin, bf16_t *wf, bf
   }
 }
 
+static inline unsigned nblk(int64_t n, int b) { return (unsigned) ((n + b - 1) / b); }
+
+/* training batch: nseg segments of seg residues starting at starts[s] of a
+ * device-resident target and its labels */
+__global__ void gate_gather_kernel(const uint8_t *codes, const float *lab, const int64_t *starts, int seg, int64_t m,
+                                   uint8_t *oc, float *ol)
+{
+  const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t src = starts[i / seg] + i % seg;
+  oc[i] = codes[src];
+  ol[i] = lab[src];
+}
+
 /* ------------------------------------------------------------------------ */
 /* C ABI (stream 0; all pointers are device pointers)                        */
 
-static inline unsigned nblk(int64_t n, int b) { return (unsigned) ((n + b - 1) / b); }
+/* Device memory for the gate's callers, from this library's runtime (the
+ * kernels below and these buffers share one runtime and its null stream, so
+ * every call is ordered after the previous one). */
+extern "C" int infx_dev_malloc(size_t bytes, void **ptr) { return (int) hipMalloc(ptr, bytes > 0 ? bytes : 1); }
+extern "C" int infx_dev_free(void *ptr) { return (int) hipFree(ptr); }
+extern "C" int infx_dev_memset(void *ptr, int value, size_t bytes) { return (int) hipMemset(ptr, value, bytes); }
+extern "C" int infx_dev_upload(void *dst, const void *src, size_t bytes) { return (int) hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }
+extern "C" int infx_dev_download(void *dst, const void *src, size_t bytes) { return (int) hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost); }
+extern "C" int infx_dev_sync(void) { return (int) hipDeviceSynchronize(); }
+
+extern "C" int infx_gate_gather(const uint8_t *codes, const float *lab, const int64_t *starts, int nseg, int seg,
+                                uint8_t *oc, float *ol)
+{
+  const int64_t m = (int64_t) nseg * seg;
+  if (m <= 0) return 0;
+  hipLaunchKernelGGL(gate_gather_kernel, dim3(nblk(m, 256)), dim3(256), 0, 0, codes, lab, starts, seg, m, oc, ol);
+  return (int) hipGetLastError();
+}
+
 
 extern "C" int infx_gate_onehot(const uint8_t *codes, int64_t n, void *x)
 {
@@ -263,25 +295,25 @@ extern "C" int infx_gate_onehot(const uint8_t *codes, int64_t n, void *x)
 }
 
 /* mode 0: bf16 out + ReLU; 1: bf16 out, no ReLU; 2: fp32 out, no ReLU */
-extern "C" int infx_gate_conv(const void *x, int64_t n, int seg, int cin, const void *w, int kp, const float *bias,
+extern "C" int infx_gate_conv(const void *x, int64_t n, int seg, int cin, int dil, const void *w, int kp, const float *bias,
                               void *y, int mode)
 {
   if (n <= 0) return 0;
-  if ((cin != 4 && cin != GATE_C) || kp % 16 || kp < GATE_KW * cin || seg <= 0) return (int) hipErrorInvalidValue;
+  if ((cin != 4 && cin != GATE_C) || kp % 16 || kp < GATE_KW * cin || seg <= 0 || dil < 1) return (int) hipErrorInvalidValue;
   const dim3 g(nblk(n, 64)), b(256);
-  if (mode == 0) hipLaunchKernelGGL((gate_conv_kernel<true, false>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, (const bf16_t *) w, kp, bias, (bf16_t *) y, (float *) nullptr);
-  else if (mode == 1) hipLaunchKernelGGL((gate_conv_kernel<false, false>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, (const bf16_t *) w, kp, bias, (bf16_t *) y, (float *) nullptr);
-  else hipLaunchKernelGGL((gate_conv_kernel<false, true>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, (const bf16_t *) w, kp, bias, (bf16_t *) nullptr, (float *) y);
+  if (mode == 0) hipLaunchKernelGGL((gate_conv_kernel<true, false>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, dil, (const bf16_t *) w, kp, bias, (bf16_t *) y, (float *) nullptr);
+  else if (mode == 1) hipLaunchKernelGGL((gate_conv_kernel<false, false>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, dil, (const bf16_t *) w, kp, bias, (bf16_t *) y, (float *) nullptr);
+  else hipLaunchKernelGGL((gate_conv_kernel<false, true>), g, b, 0, 0, (const bf16_t *) x, n, seg, cin, dil, (const bf16_t *) w, kp, bias, (bf16_t *) nullptr, (float *) y);
   return (int) hipGetLastError();
 }
 
-extern "C" int infx_gate_wgrad(const void *g, const void *x, int64_t n, int seg, int cin, int kp, float *dw)
+extern "C" int infx_gate_wgrad(const void *g, const void *x, int64_t n, int seg, int cin, int dil, int kp, float *dw)
 {
   if (n <= 0) return 0;
-  if ((cin != 4 && cin != GATE_C) || kp % 64 || kp < GATE_KW * cin || seg <= 0) return (int) hipErrorInvalidValue;
+  if ((cin != 4 && cin != GATE_C) || kp % 64 || kp < GATE_KW * cin || seg <= 0 || dil < 1) return (int) hipErrorInvalidValue;
   const int64_t tchunk = 1024;
   const dim3 grid(GATE_C / 16, kp / 64, nblk(n, (int) tchunk));
-  hipLaunchKernelGGL(gate_wgrad_kernel, grid, dim3(64), 0, 0, (const bf16_t *) g, (const bf16_t *) x, n, seg, cin, kp, tchunk, dw);
+  hipLaunchKernelGGL(gate_wgrad_kernel, grid, dim3(64), 0, 0, (const bf16_t *) g, (const bf16_t *) x, n, seg, cin, dil, kp, tchunk, dw);
   return (int) hipGetLastError();
 }
 

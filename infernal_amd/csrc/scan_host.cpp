@@ -217,11 +217,11 @@ struct StdProfile {
 
 struct SmxDev {
   std::vector<StdProfile *> prof;
-  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args;
+  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args, d_ctab;
   void  *trdev = NULL;   /* truncated-scanner resources */
   ~SmxDev() {
     for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
-    d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release();
+    d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release(); d_ctab.release();
     d_lsum_f.release(); d_lsum_i.release();
     infx_trdev_release(trdev);
   }
@@ -476,16 +476,24 @@ static bool same_profile(const StdProfile &a, const StdProfile &b)
 /* ------------------------------------------------------------------------ */
 /* the standard scanners                                                      */
 
+/* One scan over the windows wi[w]..wj[w] (sorted, disjoint) of one target:
+ * the cells, candidates and hits are those of scanning each window on its
+ * own (a chunk's DP never starts before its window), in one launch. */
 static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,
-                    int64_t i0, int64_t j0, float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
-                    int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+                    const int64_t *wi, const int64_t *wj, int nwin, float cutoff, CM_TOPHITS *hitlist, int do_null3,
+                    float env_cutoff, int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
 {
   int status;
   auto t_start = std::chrono::steady_clock::now();
   infx_timing_reset();
   /* contract checks (src/cm_dpsearch.c:129-135, :1146-1151, :1771-1776) */
+  if (nwin < 1)                       return infx_fail(eslEINCOMPAT, errbuf, "%s, no window", fname);
+  const int64_t i0 = wi[0], j0 = wj[nwin - 1];   /* the span the windows lie in */
   if (!(cm->flags & CMH_BITS))        return infx_fail(eslEINCOMPAT, errbuf, "%s, CMH_BITS flag is not raised.\n", fname);
-  if (j0 < i0)                        return infx_fail(eslEINCOMPAT, errbuf, "%s, i0: %lld j0: %lldd\n", fname, (long long) i0, (long long) j0);
+  for (int w = 0; w < nwin; w++) {
+    if (wj[w] < wi[w])                return infx_fail(eslEINCOMPAT, errbuf, "%s, i0: %lld j0: %lldd\n", fname, (long long) wi[w], (long long) wj[w]);
+    if (w > 0 && wi[w] <= wj[w - 1])  return infx_fail(eslEINCOMPAT, errbuf, "%s, windows must be sorted and disjoint", fname);
+  }
   if (dsq == NULL)                    return infx_fail(eslEINCOMPAT, errbuf, "%s, dsq is NULL\n", fname);
   if (semi == SEMI_CYK && (cm->search_opts & CM_SEARCH_INSIDE))     return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag raised", fname);
   if (semi != SEMI_CYK && !(cm->search_opts & CM_SEARCH_INSIDE))    return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag not raised", fname);
@@ -496,10 +504,15 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   if (qdbidx < 0 || qdbidx > 2)       return infx_fail(eslEINCOMPAT, errbuf, "%s, qdbidx is invalid", fname);
   if (i0 < 1)                         return infx_fail(eslEINCOMPAT, errbuf, "%s, i0 must be >= 1", fname);
   if (j0 - i0 + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
-  for (int64_t x = i0; x <= j0; x++)
-    if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
-  const int64_t L = j0 - i0 + 1;
-  const int W = (smx->W > L) ? (int) L : smx->W;
+  int64_t L = 0, Lmax = 0;      /* residues scanned; the longest window */
+  for (int w = 0; w < nwin; w++) {
+    for (int64_t x = wi[w]; x <= wj[w]; x++)
+      if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
+    L += wj[w] - wi[w] + 1;
+    Lmax = ESL_MAX(Lmax, wj[w] - wi[w] + 1);
+  }
+  const int64_t Lspan = j0 - i0 + 1;
+  const int W = (smx->W > Lmax) ? (int) Lmax : smx->W;
 
   /* profile (host only: its errors are reported with or without a device) */
   StdProfile fresh;
@@ -563,9 +576,17 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     chunk = ESL_MAX(chunk, (int64_t) 2 * W);
     chunk = ESL_MAX(chunk, (int64_t) 64);
   }
-  const int64_t nchunks64 = (L + chunk - 1) / chunk;
-  if (nchunks64 > INT32_MAX / 2) return infx_fail(eslERANGE, errbuf, "%s: too many chunks", fname);
+  int64_t nchunks64 = 0;
+  for (int w = 0; w < nwin; w++) nchunks64 += (wj[w] - wi[w] + chunk) / chunk;
+  if (nchunks64 > INT32_MAX / 4) return infx_fail(eslERANGE, errbuf, "%s: too many chunks", fname);
   const int nchunks = (int) nchunks64;
+  /* several windows: the chunk table (first row, last row, window start) */
+  std::vector<int64_t> ctab;
+  if (nwin > 1) {
+    ctab.reserve((size_t) nchunks * 3);
+    for (int w = 0; w < nwin; w++)
+      for (int64_t a = wi[w]; a <= wj[w]; a += chunk) { ctab.push_back(a); ctab.push_back(ESL_MIN(wj[w], a + chunk - 1)); ctab.push_back(wi[w]); }
+  }
   /* slot: guard | deck | guard | rows[4] (ring over j) | guard | vsc  (4-byte cells) */
   const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
   const size_t deck_off = G * 4;
@@ -580,7 +601,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
 
-  if ((status = D->d_dsq.reserve((size_t) L + 128)) != eslOK ||
+  if ((status = D->d_dsq.reserve((size_t) Lspan + 128)) != eslOK ||
       (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
       (status = D->d_chunk.reserve((size_t) nchunks * 24 + 64)) != eslOK ||
       (status = D->d_ncand.reserve(64)) != eslOK)
@@ -594,8 +615,8 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   /* target residues i0..j0 behind and before 64 zero bytes (residue codes
    * past j0 are read, and ignored, by idle lanes of the last row pair) */
-  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) L + 128, g_stream) != hipSuccess ||
-      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) L, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
+      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess)
     return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
 
   ScanArgs A;
@@ -615,6 +636,12 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB;
   A.i0 = i0; A.j0 = j0;
   A.chunk_len = chunk; A.nchunks = nchunks; A.nslots = nslots;
+  A.chunk_tab = NULL;
+  if (nwin > 1) {
+    if ((status = upload(D->d_ctab, ctab.data(), ctab.size() * sizeof(int64_t))) != eslOK)
+      return infx_fail(eslEMEM, errbuf, "%s: chunk table upload failed", fname);
+    A.chunk_tab = (const int64_t *) D->d_ctab.ptr;
+  }
   A.root_dn = P->root_dn; A.root_dx = P->root_dx; A.root_cnum = P->root_cnum;
   memcpy(A.root_child_base, P->root_child_base, sizeof(A.root_child_base));
   memcpy(A.root_tsc, P->root_tsc, sizeof(A.root_tsc));
@@ -711,7 +738,11 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   }
 
   double cells = 0;
-  infx_count_scan_cells(cm, smx, qdbidx, 0, i0, j0, &cells);
+  for (int w = 0; w < nwin; w++) {
+    double cw = 0;
+    infx_count_scan_cells(cm, smx, qdbidx, 0, wi[w], wj[w], &cw);
+    cells += cw;
+  }
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (getenv("INFX_VERBOSE"))
     fprintf(stderr, "[infx] %s: wall %.1f ms = kernel %.1f ms (%d launch(es)) + hit replay %.1f ms (%llu candidates) + rest\n", fname,
@@ -770,20 +801,33 @@ extern "C" int FastCYKScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, 
                            float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
                            int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
 {
-  return std_scan(SEMI_CYK, "FastCYKScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+  return std_scan(SEMI_CYK, "FastCYKScan", cm, errbuf, smx, qdbidx, dsq, &i0, &j0, 1, cutoff, hitlist, do_null3, env_cutoff,
                   ret_envi, ret_envj, ret_vsc, ret_sc);
 }
 extern "C" int FastFInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
                                float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
                                int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
 {
-  return std_scan(SEMI_FINSIDE, "FastFInsideScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+  return std_scan(SEMI_FINSIDE, "FastFInsideScan", cm, errbuf, smx, qdbidx, dsq, &i0, &j0, 1, cutoff, hitlist, do_null3, env_cutoff,
                   ret_envi, ret_envj, ret_vsc, ret_sc);
 }
+/* Many windows of one target in one launch (no counterpart in the
+ * reference: the windows' hits are those of one standard scan per window). */
+extern "C" int infx_FastScanWindows(int semi, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,
+                                    const int64_t *wi, const int64_t *wj, int nwin, float cutoff, CM_TOPHITS *hitlist,
+                                    int do_null3, float *ret_sc)
+{
+  static const char *names[3] = { "FastCYKScan", "FastFInsideScan", "FastIInsideScan" };
+  if (semi < SEMI_CYK || semi > SEMI_IINSIDE) return infx_fail(eslEINVAL, errbuf, "infx_FastScanWindows: bad algorithm %d", semi);
+  if (nwin < 1 || wi == NULL || wj == NULL) return infx_fail(eslEINCOMPAT, errbuf, "infx_FastScanWindows: no windows");
+  return std_scan(semi, names[semi], cm, errbuf, smx, qdbidx, dsq, wi, wj, nwin, cutoff, hitlist, do_null3, 0.f,
+                  NULL, NULL, NULL, ret_sc);
+}
+
 extern "C" int FastIInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
                                float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
                                int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
 {
-  return std_scan(SEMI_IINSIDE, "FastIInsideScan", cm, errbuf, smx, qdbidx, dsq, i0, j0, cutoff, hitlist, do_null3, env_cutoff,
+  return std_scan(SEMI_IINSIDE, "FastIInsideScan", cm, errbuf, smx, qdbidx, dsq, &i0, &j0, 1, cutoff, hitlist, do_null3, env_cutoff,
                   ret_envi, ret_envj, ret_vsc, ret_sc);
 }

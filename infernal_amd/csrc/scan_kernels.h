@@ -77,6 +77,7 @@ struct ScanArgs {
   int            M, W, R, RB;
   int64_t        i0, j0;
   int64_t        chunk_len;
+  const int64_t *chunk_tab; /* null: chunk c covers i0 + c*chunk_len ..; else [3c..3c+2] = first row, last row, window start */
   int            nchunks, nslots;
   int            acc_n;       /* LDS local-begin accumulators per wavefront */
   /* root state */

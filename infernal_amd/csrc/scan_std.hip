@@ -587,9 +587,17 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   if (VSC) for (int v = lane; v < fresh(ap)->M; v += 64) vscs[v] = IMPF;
 
   for (int c = slot; c < fresh(ap)->nchunks; c += fresh(ap)->nslots) {
-    const int64_t ac = fresh(ap)->i0 + (int64_t) c * fresh(ap)->chunk_len;
-    const int64_t bc = min(fresh(ap)->j0, ac + fresh(ap)->chunk_len - 1);
-    const int64_t s0 = max(fresh(ap)->i0, ac - (int64_t) fresh(ap)->W + 1);
+    /* chunk c reports rows ac..bc; its DP starts W-1 rows earlier, but not
+     * before the start lo of its window (one window: the whole target) */
+    int64_t ac, bc, lo;
+    if (const int64_t *tab = fresh(ap)->chunk_tab) {
+      ac = tab[3 * c]; bc = tab[3 * c + 1]; lo = tab[3 * c + 2];
+    } else {
+      ac = fresh(ap)->i0 + (int64_t) c * fresh(ap)->chunk_len;
+      bc = min(fresh(ap)->j0, ac + fresh(ap)->chunk_len - 1);
+      lo = fresh(ap)->i0;
+    }
+    const int64_t s0 = max(lo, ac - (int64_t) fresh(ap)->W + 1);
     const int R = fresh(ap)->R;
 
     {

@@ -17,7 +17,10 @@ import numpy as np
 from . import B_st, D_st, E_st, IL_st, IR_st, ML_st, MP_st, MR_st, S_st
 
 
-def markov2_background(n, gc=0.40, seed=1, seg=10000):
+def markov2_background(n, gc=0.40, seed=1, seg=10000, sample_seed=None):
+    """n residues of a second-order Markov chain with stationary GC content
+    gc; the chain's conditionals are drawn from `seed`, the sequence from
+    the same stream (or from sample_seed: another sample of the same chain)."""
     rng = np.random.default_rng(seed)
     base = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
     # per-context conditionals, then rescale them so the chain's stationary
@@ -37,6 +40,8 @@ def markov2_background(n, gc=0.40, seed=1, seg=10000):
         P = P * (base / comp)[None, :]
         P = P / P.sum(axis=1, keepdims=True)
     cdf = np.cumsum(P, axis=1)
+    if sample_seed is not None:
+        rng = np.random.default_rng(sample_seed)
     nseg = (n + seg - 1) // seg
     out = np.empty((nseg, seg), dtype=np.uint8)
     x1 = rng.choice(4, size=nseg, p=base)

@@ -1,5 +1,6 @@
 """Learned-gate kernels (infx_gate_*) against plain PyTorch fp32 references
-of the same operations on the same bf16-rounded inputs.
+of the same operations on the same bf16-rounded inputs, computed on the CPU
+(the kernels run on the library's own device buffers).
 
 Tolerances: outputs stored in bf16 carry a relative rounding of 2^-8; fp32
 accumulations over up to 1,152 products differ from torch's order by a few
@@ -22,19 +23,31 @@ def _gate():
 
 
 def _bf(x):
-    return x.to(torch.bfloat16)
+    """float tensor -> bf16-rounded float tensor (CPU)"""
+    return x.to(torch.bfloat16).float()
 
 
-def _ref_conv(x, w, cin, kp, seg, bias=None, relu=True):
+def _up(g, t, dt):
+    a = t.detach().cpu().numpy()
+    return g.DevBuf.upload(g.to_bf16_bits(a) if dt == "bf16" else a, dt)
+
+
+def _down(b):
+    from infernal_amd import gate as g
+    a = b.get()
+    return torch.from_numpy(g.from_bf16_bits(a) if b.dt == "bf16" else a.astype(np.float32))
+
+
+def _ref_conv(x, w, cin, kp, seg, bias=None, relu=True, dil=1):
     """x [n][cin] float, w [128][kp] float (kk = k*cin + ci) -> [n][128]."""
     n = x.shape[0]
     wk = w[:, :9 * cin].reshape(128, 9, cin).permute(0, 2, 1)   # [co][ci][k]
-    out = torch.empty(n, 128, device=x.device)
+    outs = []
     for a in range(0, n, seg):
         xs = x[a:a + seg].t().unsqueeze(0)                          # [1][cin][L]
-        y = torch.nn.functional.conv1d(xs, wk, bias=bias, padding=4)[0].t()
-        out[a:a + seg] = torch.relu(y) if relu else y
-    return out
+        y = torch.nn.functional.conv1d(xs, wk, bias=bias, padding=4 * dil, dilation=dil)[0].t()
+        outs.append(torch.relu(y) if relu else y)
+    return torch.cat(outs)
 
 
 def _close(got, ref, rel):
@@ -43,142 +56,141 @@ def _close(got, ref, rel):
     assert err <= rel * scale, (err, scale)
 
 
-@pytest.mark.parametrize("cin,kp,n,seg", [(4, 64, 1000, 1000), (4, 64, 1024, 256), (128, 1152, 777, 777), (128, 1152, 1024, 128)])
-def test_conv_forward(cin, kp, n, seg):
-    gate = _gate()
+@pytest.mark.parametrize("cin,kp,n,seg,dil", [(4, 64, 1000, 1000, 1), (4, 64, 1024, 256, 3), (128, 1152, 777, 777, 1),
+                                              (128, 1152, 1024, 128, 9), (128, 1152, 2000, 1000, 27)])
+def test_conv_forward(cin, kp, n, seg, dil):
+    g = _gate()
     print("case conv", cin, kp, n, seg, flush=True)
-    g = torch.Generator().manual_seed(cin + n)
-    x = _bf(torch.randn(n, cin, generator=g)).cuda()
+    gen = torch.Generator().manual_seed(cin + n)
+    x = _bf(torch.randn(n, cin, generator=gen))
     w = torch.zeros(128, kp)
-    w[:, :9 * cin] = torch.randn(128, 9 * cin, generator=g) / math.sqrt(9 * cin)
-    wb = _bf(w).cuda()
-    b = torch.randn(128, generator=g).cuda()
-    y = torch.empty(n, 128, dtype=torch.bfloat16, device="cuda")
-    gate._call("infx_gate_conv", gate._p(x), n, seg, cin, gate._p(wb), kp, gate._p(b), gate._p(y), 0)
-    ref = _ref_conv(x.float(), wb.float(), cin, kp, seg, bias=b)
-    _close(y, ref, 1.6e-2)
-    yf = torch.empty(n, 128, dtype=torch.float32, device="cuda")
-    gate._call("infx_gate_conv", gate._p(x), n, seg, cin, gate._p(wb), kp, None, gate._p(yf), 2)
-    _close(yf, _ref_conv(x.float(), wb.float(), cin, kp, seg, relu=False), 1e-4)
+    w[:, :9 * cin] = torch.randn(128, 9 * cin, generator=gen) / math.sqrt(9 * cin)
+    w = _bf(w)
+    b = torch.randn(128, generator=gen)
+    dx, dw, db = _up(g, x, "bf16"), _up(g, w, "bf16"), _up(g, b, "f32")
+    y = g.DevBuf((n, 128), "bf16")
+    g._call("infx_gate_conv", dx.ptr, n, seg, cin, dil, dw.ptr, kp, db.ptr, y.ptr, 0)
+    _close(_down(y), _ref_conv(x, w, cin, kp, seg, bias=b, dil=dil), 1.6e-2)
+    yf = g.DevBuf((n, 128), "f32")
+    g._call("infx_gate_conv", dx.ptr, n, seg, cin, dil, dw.ptr, kp, None, yf.ptr, 2)
+    _close(_down(yf), _ref_conv(x, w, cin, kp, seg, relu=False, dil=dil), 1e-4)
 
 
-def test_onehot():
-    gate = _gate()
-    codes = torch.tensor([0, 1, 2, 3, 4, 15, 2], dtype=torch.uint8, device="cuda")
-    x = torch.empty(7, 4, dtype=torch.bfloat16, device="cuda")
-    gate._call("infx_gate_onehot", gate._p(codes), 7, gate._p(x))
-    ref = torch.zeros(7, 4)
-    for i, c in enumerate([0, 1, 2, 3, 4, 15, 2]):
+def test_onehot_and_gather():
+    g = _gate()
+    codes = np.array([0, 1, 2, 3, 4, 15, 2], dtype=np.uint8)
+    dc = g.DevBuf.upload(codes, "u8")
+    x = g.DevBuf((7, 4), "bf16")
+    g._call("infx_gate_onehot", dc.ptr, 7, x.ptr)
+    ref = np.zeros((7, 4), dtype=np.float32)
+    for i, c in enumerate(codes):
         if c < 4:
             ref[i, c] = 1
-    assert torch.equal(x.float().cpu(), ref)
+    assert np.array_equal(g.from_bf16_bits(x.get()), ref)
+    big = np.arange(1000, dtype=np.int64) % 251
+    lab = (np.arange(1000) % 7).astype(np.float32)
+    st = np.array([5, 900, 0], dtype=np.int64)
+    oc, ol = g.DevBuf(3 * 64, "u8"), g.DevBuf(3 * 64, "f32")
+    # every buffer a kernel reads stays referenced until the call returns
+    db, dl, ds = g.DevBuf.upload(big, "u8"), g.DevBuf.upload(lab, "f32"), g.DevBuf.upload(st, "i64")
+    g._call("infx_gate_gather", db.ptr, dl.ptr, ds.ptr, 3, 64, oc.ptr, ol.ptr)
+    want = np.concatenate([np.arange(s, s + 64) for s in st])
+    assert np.array_equal(oc.get(), big[want].astype(np.uint8)) and np.array_equal(ol.get(), lab[want])
 
 
-@pytest.mark.parametrize("cin,kp,n,seg", [(4, 64, 3000, 1000), (128, 1152, 2048, 512)])
-def test_weight_and_data_gradients(cin, kp, n, seg):
-    gate = _gate()
+@pytest.mark.parametrize("cin,kp,n,seg,dil", [(4, 64, 3000, 1000, 1), (128, 1152, 2048, 512, 1), (128, 1152, 2048, 1024, 9)])
+def test_weight_and_data_gradients(cin, kp, n, seg, dil):
+    g = _gate()
     print("case grads", cin, kp, n, seg, flush=True)
-    g = torch.Generator().manual_seed(7 + cin)
-    x = _bf(torch.randn(n, cin, generator=g)).cuda()
-    dy = _bf(torch.randn(n, 128, generator=g)).cuda()
+    gen = torch.Generator().manual_seed(7 + cin)
+    x = _bf(torch.randn(n, cin, generator=gen))
+    dy = _bf(torch.randn(n, 128, generator=gen))
     w = torch.zeros(128, kp)
-    w[:, :9 * cin] = torch.randn(128, 9 * cin, generator=g) / math.sqrt(9 * cin)
-    w = w.cuda()
-    # autograd reference on the same bf16 values
-    xr = x.float().clone().requires_grad_(True)
-    wr = _bf(w).float().clone().requires_grad_(True)
-    yr = _ref_conv(xr, wr, cin, kp, seg, relu=False)
-    (yr * dy.float()).sum().backward()
-    dw = torch.zeros(128, kp, device="cuda")
-    gate._call("infx_gate_wgrad", gate._p(dy), gate._p(x), n, seg, cin, kp, gate._p(dw))
-    _close(dw, wr.grad, 1e-4)
+    w[:, :9 * cin] = torch.randn(128, 9 * cin, generator=gen) / math.sqrt(9 * cin)
+    xr = x.clone().requires_grad_(True)
+    wr = _bf(w).clone().requires_grad_(True)
+    (_ref_conv(xr, wr, cin, kp, seg, relu=False, dil=dil) * dy).sum().backward()
+    ddy, dxb = _up(g, dy, "bf16"), _up(g, x, "bf16")
+    dw = g.DevBuf.zeros((128, kp), "f32")
+    g._call("infx_gate_wgrad", ddy.ptr, dxb.ptr, n, seg, cin, dil, kp, dw.ptr)
+    got = _down(dw)
+    _close(got, wr.grad, 1e-4)
     if 9 * cin < kp:
-        assert dw[:, 9 * cin:].abs().max().item() == 0.0
+        assert got[:, 9 * cin:].abs().max().item() == 0.0
     if cin == 128:
-        wf = torch.zeros(128, kp, dtype=torch.bfloat16, device="cuda")
-        wd = torch.zeros(128, kp, dtype=torch.bfloat16, device="cuda")
-        gate._call("infx_gate_pack", gate._p(w), kp, cin, gate._p(wf), gate._p(wd))
-        assert torch.equal(wf, _bf(w))
-        dx = torch.empty(n, 128, dtype=torch.bfloat16, device="cuda")
-        gate._call("infx_gate_conv", gate._p(dy), n, seg, 128, gate._p(wd), kp, None, gate._p(dx), 1)
-        _close(dx, xr.grad, 1.6e-2)
-    cs = torch.zeros(128, device="cuda")
-    gate._call("infx_gate_colsum", gate._p(dy), n, gate._p(cs))
-    _close(cs, dy.float().sum(0), 1e-4)
+        wm = _up(g, w, "f32")
+        wf = g.DevBuf.zeros((128, kp), "bf16")
+        wd = g.DevBuf.zeros((128, kp), "bf16")
+        g._call("infx_gate_pack", wm.ptr, kp, cin, wf.ptr, wd.ptr)
+        assert torch.equal(_down(wf), _bf(w))
+        dxo = g.DevBuf((n, 128), "bf16")
+        g._call("infx_gate_conv", ddy.ptr, n, seg, 128, dil, wd.ptr, kp, None, dxo.ptr, 1)
+        _close(_down(dxo), xr.grad, 1.6e-2)
+    cs = g.DevBuf.zeros(128, "f32")
+    g._call("infx_gate_colsum", ddy.ptr, n, cs.ptr)
+    _close(_down(cs), dy.sum(0), 1e-4)
 
 
 def test_head_loss_relu_adam():
-    gate = _gate()
+    g = _gate()
     n = 5000
-    g = torch.Generator().manual_seed(3)
-    y = _bf(torch.relu(torch.randn(n, 128, generator=g))).cuda()
-    hw = torch.randn(128, generator=g).cuda()
-    hb = torch.randn(1, generator=g).cuda()
-    s = torch.empty(n, device="cuda")
-    gate._call("infx_gate_head", gate._p(y), n, gate._p(hw), gate._p(hb), gate._p(s))
-    sref = y.float() @ hw + hb
-    _close(s, sref, 1e-4)
-    lab = (torch.rand(n, generator=g) < 0.2).float().cuda()
+    gen = torch.Generator().manual_seed(3)
+    y = _bf(torch.relu(torch.randn(n, 128, generator=gen)))
+    hw = torch.randn(128, generator=gen)
+    hb = torch.randn(1, generator=gen)
+    dyb, dhw_in, dhb_in = _up(g, y, "bf16"), _up(g, hw, "f32"), _up(g, hb, "f32")
+    s = g.DevBuf(n, "f32")
+    g._call("infx_gate_head", dyb.ptr, n, dhw_in.ptr, dhb_in.ptr, s.ptr)
+    st = _down(s)
+    _close(st, y @ hw + hb, 1e-4)
+    lab = (torch.rand(n, generator=gen) < 0.2).float()
     pw = 3.0
-    dy = torch.empty(n, 128, dtype=torch.bfloat16, device="cuda")
-    dhw = torch.zeros(128, device="cuda")
-    dhb = torch.zeros(1, device="cuda")
-    loss = torch.zeros(1, device="cuda")
-    gate._call("infx_gate_head_bwd", gate._p(s), gate._p(lab), gate._p(y), n, gate._p(hw), pw, gate._p(dy),
-               gate._p(dhw), gate._p(dhb), gate._p(loss))
-    sr = s.clone().requires_grad_(True)
+    dy = g.DevBuf((n, 128), "bf16")
+    dhw, dhb, loss = g.DevBuf.zeros(128, "f32"), g.DevBuf.zeros(1, "f32"), g.DevBuf.zeros(1, "f32")
+    dlab = _up(g, lab, "f32")
+    g._call("infx_gate_head_bwd", s.ptr, dlab.ptr, dyb.ptr, n, dhw_in.ptr, pw, dy.ptr,
+            dhw.ptr, dhb.ptr, loss.ptr)
+    sr = st.clone().requires_grad_(True)
     wgt = torch.where(lab > 0.5, torch.full_like(lab, pw), torch.ones_like(lab))
     lref = (wgt * torch.nn.functional.binary_cross_entropy_with_logits(sr, lab, reduction="none")).mean()
     lref.backward()
-    assert abs(loss.item() - lref.item()) <= 1e-4 * max(1.0, lref.item())
-    _close(dhw, y.float().t() @ sr.grad, 1e-3)
-    assert abs(dhb.item() - sr.grad.sum().item()) <= 1e-4
-    _close(dy, sr.grad[:, None] * hw[None, :], 1.6e-2)
-    # ReLU backward
-    g2 = dy.clone()
-    gate._call("infx_gate_relu_bwd", gate._p(g2), gate._p(y), n * 128)
-    assert torch.equal(g2, torch.where(y.float() > 0, dy, torch.zeros_like(dy)))
-    # Adam
-    w = torch.randn(1000, generator=g).cuda()
-    gr = torch.randn(1000, generator=g).cuda()
-    m = torch.zeros(1000, device="cuda")
-    v = torch.zeros(1000, device="cuda")
-    w0 = w.clone()
-    gate._call("infx_gate_adam", gate._p(w), gate._p(gr), gate._p(m), gate._p(v), 1000, 0.01, 0.9, 0.999, 1e-8, 1)
+    assert abs(_down(loss).item() - lref.item()) <= 1e-4 * max(1.0, lref.item())
+    _close(_down(dhw), y.t() @ sr.grad, 1e-3)
+    assert abs(_down(dhb).item() - sr.grad.sum().item()) <= 1e-4
+    dyt = _down(dy)
+    _close(dyt, sr.grad[:, None] * hw[None, :], 1.6e-2)
+    # ReLU backward: keeps the gradient where the activation is positive
+    g._call("infx_gate_relu_bwd", dy.ptr, dyb.ptr, n * 128)
+    assert torch.equal(_down(dy), torch.where(y > 0, dyt, torch.zeros_like(dyt)))
+    # Adam, first step
+    w0 = torch.randn(1000, generator=gen)
+    gr = torch.randn(1000, generator=gen)
+    w, m, v = _up(g, w0, "f32"), g.DevBuf.zeros(1000, "f32"), g.DevBuf.zeros(1000, "f32")
+    dgr = _up(g, gr, "f32")
+    g._call("infx_gate_adam", w.ptr, dgr.ptr, m.ptr, v.ptr, 1000, 0.01, 0.9, 0.999, 1e-8, 1)
     mref, vref = 0.1 * gr, 0.001 * gr * gr
     wref = w0 - 0.01 * (mref / 0.1) / (torch.sqrt(vref / 0.001) + 1e-8)
-    _close(w, wref, 1e-5)
+    _close(_down(w), wref, 1e-5)
 
 
 def test_training_learns_a_motif():
     """A planted 12-nt motif in random sequence: after 150 steps the gate
     ranks motif residues above the rest (AUC > 0.95)."""
-    gate = _gate()
+    g = _gate()
     rng = np.random.default_rng(0)
     motif = rng.integers(0, 4, 12).astype(np.uint8)
-    net = gate.GateNet(seed=1)
-    seg, nseg = 512, 16
-
-    def batch():
-        codes = rng.integers(0, 4, (nseg, seg)).astype(np.uint8)
-        lab = np.zeros((nseg, seg), dtype=np.float32)
-        for s in range(nseg):
-            for _ in range(3):
-                p = int(rng.integers(20, seg - 32))
-                codes[s, p:p + 12] = motif
-                lab[s, p:p + 12] = 1.0
-        return torch.from_numpy(codes.ravel()).cuda(), torch.from_numpy(lab.ravel()).cuda(), lab.ravel()
-
-    first = None
-    for it in range(150):
-        c, l, _ = batch()
-        loss = net.train_step(c, l, seg, lr=2e-3, pos_weight=4.0)
-        first = loss if first is None else first
-    c, l, lab = batch()
-    s, _ = net.forward(c, seg)
-    s = s.cpu().numpy()
-    pos, neg = s[lab > 0.5], s[lab < 0.5]
+    n = 200_000
+    codes = rng.integers(0, 4, n).astype(np.uint8)
+    lab = np.zeros(n, dtype=np.float32)
+    for p in rng.integers(0, n - 12, 600):
+        codes[p:p + 12] = motif
+        lab[p:p + 12] = 1.0
+    net = g.GateNet(seed=1)
+    losses = g.train(net, codes[:150_000], lab[:150_000], 150, seg=512, nseg=16, lr=2e-3, pos_weight=4.0)
+    s = net.score(codes[150_000:])
+    t = lab[150_000:]
+    pos, neg = s[t > 0.5], s[t < 0.5]
     auc = (pos[:, None] > neg[None, ::50]).mean()
-    print("loss", first, "->", loss, "auc", auc)
-    assert loss < 0.5 * first
+    print("loss", losses[0], "->", losses[-1], "auc", auc)
+    assert losses[-1] < 0.5 * losses[0]
     assert auc > 0.95

@@ -136,3 +136,30 @@ def test_synthetic_1500nt_cyk():
     dsq = _planted("synth_nested1500.cm", 5000, 2, seed=31)
     g, _ = _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 5000, 0.0, vsc=True)
     assert g["sc"] > 50.0
+
+
+@pytest.mark.parametrize("algo", ["cyk", "finside"])
+def test_scan_windows_equals_one_scan_per_window(algo):
+    """infx_FastScanWindows: windows of 40..3000 residues (some shorter than
+    W) in one launch return exactly the hits of one scan per window, and the
+    oracle's."""
+    cm = load_cm("synth_multiloop350.cm")
+    dsq = _planted("synth_multiloop350.cm", 20000, 6, seed=41)
+    wins = [(5, 44), (100, 900), (1200, 4200), (4300, 4310), (6000, 7999), (9000, 12000), (15000, 15500), (19500, 20000)]
+    qdb, fn, oa = (1, ia.FastCYKScan, ob.ORACLE_CYK) if algo == "cyk" else (2, ia.FastFInsideScan, ob.ORACLE_FINSIDE)
+    print("case windows", algo, len(wins), flush=True)
+    ia.set_launch_params(chunk_len=500)    # several chunks per long window
+    try:
+        got = ia.ScanWindows(algo, cm, qdb, dsq, wins, -5.0)
+    finally:
+        ia.set_launch_params(0, 0)
+    want, best = [], -1e30
+    for (i, j) in wins:
+        r = fn(cm, qdb, dsq, i, j, -5.0)
+        o = ob.scan(cm, oa, qdb, dsq, i, j, -5.0)
+        assert hits_equal(r["hits"], o["hits"]), describe_hit_diff(r["hits"], o["hits"])
+        want += r["hits"]
+        best = max(best, r["sc"])
+    key = lambda h: (h[0], h[1])
+    assert hits_equal(sorted(got["hits"], key=key), sorted(want, key=key))
+    assert np.float32(got["sc"]).view(np.uint32) == np.float32(best).view(np.uint32)
