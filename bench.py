@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8000, help="residues per CPU-baseline window")
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--gate-steps", type=int, default=400, help="learned-gate training steps (0 = skip the gate leg)")
+    ap.add_argument("--gate-steps", type=int, default=6000, help="learned-gate training steps (0 = skip the gate leg)")
     ap.add_argument("--gate-train-mb", type=float, default=8.0, help="learned-gate training target in megabases")
     ap.add_argument("--gate-smooth", type=int, default=128, help="residues the gate's logits are averaged over")
     ap.add_argument("--gate-margin", type=float, default=0.5, help="logit margin below the threshold quantile")
