@@ -462,7 +462,7 @@ int oracle_scan(const CM_t *cm, int algo, int qdbidx, int pass_idx, const ESL_DS
 
 void oracle_result_free(ORACLE_RESULT *res)
 {
-  free(res->hits); free(res->vsc); free(res->rows_sc); free(res->rows_r); free(res->rows_mode);
+  free(res->hits); free(res->vsc); free(res->rows_sc); free(res->rows_r); free(res->rows_mode); free(res->rows_msc);
   memset(res, 0, sizeof(*res));
 }
 

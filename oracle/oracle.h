@@ -43,6 +43,10 @@ typedef struct oracle_result_s {
   float      *rows_sc;       /* [(j-i0)*(W+1) + d] */
   int        *rows_r;
   char       *rows_mode;
+  /* truncated scanners: per (j, d) and mode (TRMODE_T/R/L/J = 0..3) the
+   * best truncated begin max_y X_mode[y][d] + penalty[y], before the root's
+   * record rule (IMPOSSIBLE where no begin reaches the cell) */
+  float      *rows_msc;      /* [((j-i0)*(W+1) + d)*4 + mode] */
 } ORACLE_RESULT;
 
 /* Run one scan.  algo: ORACLE_*; greedy: 1 greedy overlap removal, 0 the

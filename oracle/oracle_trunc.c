@@ -162,6 +162,8 @@ int oracle_trcyk(const CM_t *cm, int qdbidx, int pass_idx, const ESL_DSQ *dsq, i
     res->rows_sc = (float *) malloc(sizeof(float) * (size_t) L * (W + 1));
     res->rows_r = (int *) malloc(sizeof(int) * (size_t) L * (W + 1));
     res->rows_mode = (char *) malloc((size_t) L * (W + 1));
+    res->rows_msc = (float *) malloc(sizeof(float) * (size_t) L * (W + 1) * 4);
+    for (size_t x = 0; x < (size_t) L * (W + 1) * 4; x++) res->rows_msc[x] = IMPOSSIBLE;
   }
   int status = eslOK;
 
@@ -335,6 +337,14 @@ int oracle_trcyk(const CM_t *cm, int qdbidx, int pass_idx, const ESL_DSQ *dsq, i
         int dx = ESL_MIN(dxA[0], dxA[y]);
         int jp_y = cur;
         float sc;
+        if (want_rows)
+          for (d = dn; d <= dx; d++) {
+            float *m = &res->rows_msc[((size_t) (jp_g - 1) * (W + 1) + d) * 4];
+            m[TRMODE_J] = ESL_MAX(m[TRMODE_J], X(J, jp_y, y, d) + trpenalty);
+            if (fill_L) m[TRMODE_L] = ESL_MAX(m[TRMODE_L], X(Lm, jp_y, y, d) + trpenalty);
+            if (fill_R) m[TRMODE_R] = ESL_MAX(m[TRMODE_R], X(R, jp_y, y, d) + trpenalty);
+            if (fill_T && cm->sttype[y] == B_st) m[TRMODE_T] = ESL_MAX(m[TRMODE_T], X(T, jp_y, y, d) + trpenalty);
+          }
         for (d = dn; d <= dx; d++) {
           sc = X(J, jp_y, y, d) + trpenalty;
           if (sc > X(J, jp_v, 0, d)) { X(J, jp_v, 0, d) = sc; if (sc > bestsc[d]) { bestsc[d] = sc; bestmode[d] = TRMODE_J; bestr[d] = y; } }
@@ -463,6 +473,8 @@ int oracle_triinside(const CM_t *cm, int qdbidx, int pass_idx, const ESL_DSQ *ds
     res->rows_sc = (float *) malloc(sizeof(float) * (size_t) L * (W + 1));
     res->rows_r = (int *) malloc(sizeof(int) * (size_t) L * (W + 1));
     res->rows_mode = (char *) malloc((size_t) L * (W + 1));
+    res->rows_msc = (float *) malloc(sizeof(float) * (size_t) L * (W + 1) * 4);
+    for (size_t x = 0; x < (size_t) L * (W + 1) * 4; x++) res->rows_msc[x] = IMPOSSIBLE;
   }
   int status = eslOK;
 
@@ -639,6 +651,14 @@ int oracle_triinside(const CM_t *cm, int qdbidx, int pass_idx, const ESL_DSQ *ds
         int dx = ESL_MIN(dxA[0], dxA[y]);
         int jp_y = cur, sc;
         float fsc;
+        if (want_rows)
+          for (d = dn; d <= dx; d++) {
+            float *m = &res->rows_msc[((size_t) (jp_g - 1) * (W + 1) + d) * 4];
+            m[TRMODE_J] = ESL_MAX(m[TRMODE_J], Scorify_t(IADD(X(J, jp_y, y, d), itr)));
+            if (fill_L) m[TRMODE_L] = ESL_MAX(m[TRMODE_L], Scorify_t(IADD(X(Lm, jp_y, y, d), itr)));
+            if (fill_R) m[TRMODE_R] = ESL_MAX(m[TRMODE_R], Scorify_t(IADD(X(R, jp_y, y, d), itr)));
+            if (fill_T && cm->sttype[y] == B_st) m[TRMODE_T] = ESL_MAX(m[TRMODE_T], Scorify_t(IADD(X(T, jp_y, y, d), itr)));
+          }
         for (d = dn; d <= dx; d++) {
           sc = IADD(X(J, jp_y, y, d), itr);
           if (sc > X(J, jp_v, 0, d)) { X(J, jp_v, 0, d) = sc; fsc = Scorify_t(sc); if (fsc > bestsc[d]) { bestsc[d] = fsc; bestmode[d] = TRMODE_J; bestr[d] = y; } }

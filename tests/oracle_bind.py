@@ -22,7 +22,8 @@ class ORACLE_HIT(C.Structure):
 class ORACLE_RESULT(C.Structure):
     _fields_ = [("nhits", C.c_int), ("hits", C.POINTER(ORACLE_HIT)), ("envi", C.c_int64), ("envj", C.c_int64),
                 ("sc", C.c_float), ("mode", C.c_int), ("vsc", C.POINTER(C.c_float)), ("W", C.c_int),
-                ("rows_sc", C.POINTER(C.c_float)), ("rows_r", C.POINTER(C.c_int)), ("rows_mode", C.POINTER(C.c_char))]
+                ("rows_sc", C.POINTER(C.c_float)), ("rows_r", C.POINTER(C.c_int)), ("rows_mode", C.POINTER(C.c_char)),
+                ("rows_msc", C.POINTER(C.c_float))]
 
 
 _lib = None
@@ -65,5 +66,9 @@ def scan(cm, algo, qdbidx, dsq, i0, j0, cutoff, do_null3=False, greedy=True, env
         out["W"] = res.W
         out["rows_sc"] = np.ctypeslib.as_array(res.rows_sc, shape=(n,)).copy().reshape(j0 - i0 + 1, res.W + 1)
         out["rows_r"] = np.ctypeslib.as_array(res.rows_r, shape=(n,)).copy().reshape(j0 - i0 + 1, res.W + 1)
+        if res.rows_mode:
+            out["rows_mode"] = np.frombuffer(C.string_at(res.rows_mode, n), dtype=np.uint8).copy().reshape(j0 - i0 + 1, res.W + 1)
+        if res.rows_msc:
+            out["rows_msc"] = np.ctypeslib.as_array(res.rows_msc, shape=(n * 4,)).copy().reshape(j0 - i0 + 1, res.W + 1, 4)
     lib.oracle_result_free(C.byref(res))
     return out
