@@ -76,6 +76,10 @@ def cpu_baseline(cm, codes, threads, sample_len, W):
         for s in starts:
             cells += _cells(cm, q, int(s), int(s) + Lw - 1)
         res[name] = (cells, wall)
+    # one core: the first window alone
+    s0 = int(starts[0])
+    for name, algo, q in (("cyk", ob.ORACLE_CYK, 1), ("finside", ob.ORACLE_FINSIDE, 2)):
+        res[name + "_1core"] = (_cells(cm, q, s0, s0 + Lw - 1), one(algo, q, s0))
     return res
 
 
@@ -118,8 +122,83 @@ def cyk_1500(args, ia, synth, threads):
         cells = sum(_cells(cm, 1, int(s), int(s) + Lw - 1) for s in starts)
         out["cpu_gcells_per_s"] = round(cells / cw / 1e9, 4)
         out["cpu_cores"] = threads
+        s0 = int(starts[0])
+        t1 = time.perf_counter()
+        one(s0)
+        c1 = time.perf_counter() - t1
+        out["cpu_1core_gcells_per_s"] = round(_cells(cm, 1, s0, s0 + Lw - 1) / c1 / 1e9, 4)
         out["cpu_sample"] = "oracle FastCYKScan, %d windows of %d nt in %d threads, %.1f s" % (threads, Lw, threads, cw)
         out["gpu_over_cpu"] = round(out["gpu_gcells_per_s"] / out["cpu_gcells_per_s"], 1)
+        out["gpu_over_cpu_1core"] = round(out["gpu_gcells_per_s"] / out["cpu_1core_gcells_per_s"], 1)
+    return out
+
+
+def trunc_leg(args, ia, synth, threads):
+    """Truncated scanning with the 1,500-nt model: --trunc-windows windows of
+    200-2,000 nt (uniform), half an emitted homolog clipped by the window at
+    a random offset, half random ACGU decoys.  RefTrCYKScan (QDB1) and
+    RefITrInsideScan (QDB2) in the 5'-and-3' truncation pass (J, L, R and T
+    modes filled), all windows in one launch each; the oracle on the host
+    scans a sample of the same windows."""
+    path = os.path.join(ROOT, "tests", "golden", "cm", "synth_nested1500.cm")
+    cm = ia.CM.read(path)
+    cm.configure(local=True, trunc=True, scanmx=True, trscanmx=True)
+    codes, wins, hom = synth.truncated_windows(ia.CM.read(path), args.trunc_windows, seed=33, pool=50)
+    dsq = ia.dsq_from_codes(codes)
+    pidx = ia.PLI_PASS_5P_AND_3P_FORCE
+    out = {"cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), cm.clen, cm.M, cm.W),
+           "windows": len(wins), "residues": int(len(codes)), "pass": "5P_AND_3P_FORCE",
+           "note": "cells are banded (v, j, d) cells; each is computed in J, L and R mode (T too for bifurcations)"}
+    # launches of --trunc-batch windows of similar length (one wavefront per
+    # window: a launch lasts as long as its longest window)
+    order = sorted(range(len(wins)), key=lambda k: wins[k][1] - wins[k][0])
+    batches = [order[a:a + args.trunc_batch] for a in range(0, len(order), args.trunc_batch)]
+    out["launch_windows"] = args.trunc_batch
+    for algo, q in (("cyk", 1), ("iinside", 2)):
+        ia.TrScanWindows(algo, cm, q, pidx, dsq, [wins[order[0]]], 0.0)    # first-use set-up
+        wall = kern = cells = 0.0
+        hitw = [False] * len(wins)
+        nh = 0
+        for b in batches:
+            t0 = time.perf_counter()
+            r = ia.TrScanWindows(algo, cm, q, pidx, dsq, [wins[k] for k in b], 0.0)
+            wall += time.perf_counter() - t0
+            tm = ia.last_call_timing()
+            kern += tm["kernel_ms"] / 1e3
+            cells += tm["cells"]
+            for k, h in zip(b, r["hits"]):
+                hitw[k] = len(h) > 0
+                nh += len(h)
+            log("truncated %s: launch of %d windows (%d-%d nt) %.2f s kernel" % (algo, len(b), wins[b[0]][1] - wins[b[0]][0] + 1,
+                                                                                 wins[b[-1]][1] - wins[b[-1]][0] + 1, tm["kernel_ms"] / 1e3))
+        log("truncated %s: %.2f s wall, %.2f s kernel, %d windows, %d hits" % (algo, wall, kern, len(wins), nh))
+        out[algo] = {"wall_s": round(wall, 3), "kernel_s": round(kern, 3), "cells": cells,
+                     "gpu_gcells_per_s": round(cells / wall / 1e9, 3),
+                     "gpu_kernel_gcells_per_s": round(cells / kern / 1e9, 3),
+                     "homolog_windows_hit": int(sum(h and w for h, w in zip(hitw, hom))),
+                     "decoy_windows_hit": int(sum(h and not w for h, w in zip(hitw, hom)))}
+    if threads:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind as ob
+        ob.load()
+        rng = np.random.default_rng(77)
+        sample = [wins[k] for k in rng.choice(len(wins), min(threads, len(wins)), replace=False)]
+
+        def one(w):
+            ob.scan(cm, ob.ORACLE_TRCYK, 1, dsq, int(w[0]), int(w[1]), 1e30, want_rows=False, pass_idx=pidx)
+
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(len(sample)) as ex:
+            list(ex.map(one, sample))
+        cw = time.perf_counter() - t0
+        cells = sum(_cells(cm, 1, int(w[0]), int(w[1]), truncated=True) for w in sample)
+        t1 = time.perf_counter()
+        one(sample[0])
+        c1 = time.perf_counter() - t1
+        out["cpu"] = {"cyk_gcells_per_s": round(cells / cw / 1e9, 4), "cores": len(sample),
+                      "cyk_1core_gcells_per_s": round(_cells(cm, 1, int(sample[0][0]), int(sample[0][1]), truncated=True) / c1 / 1e9, 4),
+                      "sample": "oracle RefTrCYKScan on %d of the windows in %d threads, %.1f s" % (len(sample), len(sample), cw)}
+        out["gpu_over_cpu_cyk"] = round(out["cyk"]["gpu_gcells_per_s"] / out["cpu"]["cyk_gcells_per_s"], 1)
     return out
 
 
@@ -198,13 +277,14 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
                       "margin": args.gate_margin, "quantile": args.gate_quantile}}
 
 
-def _cells(cm, qdbidx, i0, j0):
+def _cells(cm, qdbidx, i0, j0, truncated=False):
     import ctypes as C
     import infernal_amd as ia
     lib = ia.lib()
     lib.infx_count_scan_cells.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     out = C.c_double()
-    lib.infx_count_scan_cells(C.cast(cm.ptr, C.c_void_p), cm.c.smx, qdbidx, 0, i0, j0, C.byref(out))
+    mx = cm.c.trsmx if truncated else cm.c.smx
+    lib.infx_count_scan_cells(C.cast(cm.ptr, C.c_void_p), mx, qdbidx, 1 if truncated else 0, i0, j0, C.byref(out))
     return out.value
 
 
@@ -228,6 +308,9 @@ def main():
     ap.add_argument("--gate-quantile", type=float, default=0.005, help="held-out hit quantile the threshold sits at")
     ap.add_argument("--cyk1500-mb", type=float, default=20.0,
                     help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
+    ap.add_argument("--trunc-windows", type=int, default=32,
+                    help="windows of the truncated-scan leg (1,500-nt model, 200-2,000 nt each); 0 = skip")
+    ap.add_argument("--trunc-batch", type=int, default=32, help="windows per truncated-scan launch")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -332,7 +415,11 @@ def main():
                "sample": "oracle FastCYKScan (QDB1) + FastFInsideScan (QDB2), %d windows of %d nt in %d threads, "
                          "%.1f s CYK + %.1f s FInside" % (threads, args.cpu_sample + W - 1, threads, r["cyk"][1], r["finside"][1]),
                "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
-               "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4)}
+               "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4),
+               "one_core": {"value": round((r["cyk_1core"][0] + r["finside_1core"][0]) / (r["cyk_1core"][1] + r["finside_1core"][1]) / 1e9, 4),
+                            "cyk_gcells_per_s": round(r["cyk_1core"][0] / r["cyk_1core"][1] / 1e9, 4),
+                            "finside_gcells_per_s": round(r["finside_1core"][0] / r["finside_1core"][1] / 1e9, 4),
+                            "sample": "the first of those windows, alone"}}
 
     gate = None
     if args.gate_steps > 0 and args.only == "both":
@@ -341,6 +428,10 @@ def main():
     big = None
     if args.cyk1500_mb > 0:
         big = cyk_1500(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
+
+    trunc = None
+    if args.trunc_windows > 0:
+        trunc = trunc_leg(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
 
     out = {
         "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
@@ -364,6 +455,7 @@ def main():
         "learned_gate_speedup": (gate["speedup"] if gate and gate.get("recall", 0) >= 0.99 else None),
         "learned_gate": gate,
         "cyk_1500nt": big,
+        "truncated_1500nt": trunc,
     }
     print(json.dumps(out))
 

@@ -333,6 +333,47 @@ def ScanWindows(algo, cm, qdbidx, dsq, windows, cutoff, do_null3=False, greedy=T
     return {"hits": hits, "sc": sc.value}
 
 
+def TrScanWindows(algo, cm, qdbidx, pass_idx, dsq, windows, cutoff, do_null3=False, env_cutoff=0.0, want_hits=True,
+                  greedy=True):
+    """infx_TrScanWindows: one truncated scan per window [(i, j), ...] of one
+    target, all in one launch (one wavefront per window); algo "cyk"
+    (RefTrCYKScan) or "iinside" (RefITrInsideScan).  Per window: best score,
+    its mode, envelope and (want_hits) hit list, as one call per window."""
+    inside = {"cyk": 0, "iinside": 1}[algo]
+    c = cm.c
+    if inside:
+        c.search_opts |= CM_SEARCH_INSIDE
+    else:
+        c.search_opts &= ~CM_SEARCH_INSIDE
+    if greedy:
+        c.search_opts &= ~CM_SEARCH_CMNOTGREEDY
+    else:
+        c.search_opts |= CM_SEARCH_CMNOTGREEDY
+    dsq = np.ascontiguousarray(dsq, dtype=np.uint8)
+    n = len(windows)
+    wi = np.ascontiguousarray([w[0] for w in windows], dtype=np.int64)
+    wj = np.ascontiguousarray([w[1] for w in windows], dtype=np.int64)
+    ths = [_lib.cm_tophits_Create() for _ in range(n)] if want_hits else []
+    arr = (C.c_void_p * n)(*[C.cast(t, C.c_void_p) for t in ths]) if want_hits else None
+    sc = np.zeros(n, np.float32)
+    md = np.zeros(n, np.int8)
+    ei = np.zeros(n, np.int64)
+    ej = np.zeros(n, np.int64)
+    f = _lib.infx_TrScanWindows
+    f.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_float, C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    eb = C.create_string_buffer(ERRBUF)
+    st = f(C.cast(cm.ptr, C.c_void_p), eb, C.cast(c.trsmx, C.c_void_p), inside, qdbidx, pass_idx, dsq.ctypes.data, n,
+           wi.ctypes.data, wj.ctypes.data, cutoff, arr, int(do_null3), env_cutoff, sc.ctypes.data, md.ctypes.data,
+           ei.ctypes.data, ej.ctypes.data)
+    hits = [_hits_list(t) for t in ths] if (want_hits and st == eslOK) else None
+    for t in ths:
+        _lib.cm_tophits_Destroy(t)
+    if st != eslOK:
+        raise ScanError(st, eb.value.decode(errors="replace"))
+    return {"sc": sc, "mode": md, "envi": ei, "envj": ej, "hits": hits}
+
+
 def FastCYKScan(cm, qdbidx, dsq, i0, j0, cutoff, do_null3=False, env_cutoff=0.0, want_vsc=False, want_hits=True, greedy=True):
     """src/cm_dpsearch.c:90 FastCYKScan on the GPU."""
     return _scan("FastCYKScan", cm, qdbidx, dsq, i0, j0, cutoff, do_null3, env_cutoff, want_vsc, want_hits, greedy=greedy)

@@ -15,10 +15,10 @@
 #include "scan_kernels.h"
 #include "scan_host.h"
 
-extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, dim3 grid, dim3 block, hipStream_t stream,
-                                      const ScanArgs *a, int acc_n);
-extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n);
-extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n);
+extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int gacc, int wide, dim3 grid, dim3 block,
+                                      hipStream_t stream, const ScanArgs *a, int acc_n);
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc);
+extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
 
 /* ------------------------------------------------------------------------ */
 /* scan matrices (src/cm_mx.c:5686-5836, :6391-6560)                          */
@@ -554,15 +554,49 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   /* launch shape: wavefronts per workgroup (LDS: log-sum table + per-wave
    * local-begin accumulators), workgroups resident per CU */
   const int acc_n = (P->root_dn <= P->root_dx) ? P->root_dx - P->root_dn + 1 : 0;
-  /* the workgroup size that keeps the most wavefronts resident (ties: larger) */
   int tbl_lds = 1;
   if (const char *e = getenv("INFX_LOGSUM_TABLE")) tbl_lds = strcmp(e, "l2") != 0;
-  int wpb = 0, occ = 0;
-  for (int w = 16; w >= 1; w >>= 1) {
-    if (infx_wpb_override() && w != infx_wpb_override()) continue;
-    if (infx_std_lds_bytes(semi, tbl_lds, w, acc_n) > (size_t) g_lds_max) continue;
-    const int o = infx_std_occupancy(semi, ret_vsc != NULL, tbl_lds, w, acc_n);
-    if (o * w > occ * wpb) { wpb = w; occ = o; }
+  /* accumulators in LDS, or in the global slot when that keeps more
+   * wavefronts resident (wide root bands) */
+  int gacc_mode = -1;   /* -1: choose, 0: LDS, 1: global */
+  if (const char *e = getenv("INFX_BEGIN_ACC")) gacc_mode = (strcmp(e, "global") == 0) ? 1 : (strcmp(e, "lds") == 0) ? 0 : -1;
+  /* chunks of the shortest length (2W) the target gives: more resident
+   * wavefronts than that find no work */
+  int64_t units = 0;
+  for (int w = 0; w < nwin; w++) units += (wj[w] - wi[w] + 1 + 2 * (int64_t) W - 1) / (2 * (int64_t) W);
+  auto pick = [&](int wide, int *wpb_, int *occ_, int *gacc_) {
+    int wpb = 0, occ = 0, gacc = 0;
+    for (int g = 0; g <= 1; g++) {
+      if ((gacc_mode >= 0 && g != gacc_mode) || (g && acc_n == 0)) continue;
+      int bw = 0, bo = 0;   /* the workgroup size that keeps the most wavefronts resident (ties: larger) */
+      for (int w = 16; w >= 1; w >>= 1) {
+        if (infx_wpb_override() && w != infx_wpb_override()) continue;
+        if (infx_std_lds_bytes(semi, tbl_lds, w, acc_n, g) > (size_t) g_lds_max) continue;
+        const int o = infx_std_occupancy(semi, ret_vsc != NULL, tbl_lds, w, acc_n, g, wide);
+        if (o * w > bo * bw) { bw = w; bo = o; }
+      }
+      if (bw == 0 || bo == 0) continue;
+      /* global accumulators cost an L2 atomic per begin cell: taken only for
+       * at least 5/4 of the busy wavefronts LDS accumulators allow */
+      const int64_t busy_new = ESL_MIN(units, (int64_t) bo * bw * g_ncu), busy_old = ESL_MIN(units, (int64_t) occ * wpb * g_ncu);
+      if (wpb == 0 || busy_new * 4 >= busy_old * 5) { wpb = bw; occ = bo; gacc = g; }
+    }
+    *wpb_ = wpb; *occ_ = occ; *gacc_ = gacc;
+  };
+  int wpb = 0, occ = 0, gacc = 0, wide = 0;
+  pick(0, &wpb, &occ, &gacc);
+  /* wide-band CYK (four blocks of loads in flight per wavefront) when the
+   * target gives fewer chunks of 2W residues than half the wavefronts the
+   * narrow kernel keeps resident: fewer, busier wavefronts */
+  {
+    int wide_mode = -1;
+    if (const char *e = getenv("INFX_WIDE")) wide_mode = atoi(e) ? 1 : 0;
+    const int want = (semi == SEMI_CYK) && (wide_mode == 1 || (wide_mode < 0 && units * 2 < (int64_t) occ * wpb * g_ncu));
+    if (want) {
+      int w2 = 0, o2 = 0, g2 = 0;
+      pick(1, &w2, &o2, &g2);
+      if (w2 > 0 && o2 > 0) { wpb = w2; occ = o2; gacc = g2; wide = 1; }
+    }
   }
   if (wpb < 1 || occ < 1)
     return infx_fail(eslEINVAL, errbuf, "%s: no workgroup shape fits (W = %d, LDS limit %zu bytes)", fname, W, (size_t) g_lds_max);
@@ -592,7 +626,8 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   const size_t deck_off = G * 4;
   const size_t row_off  = deck_off + ((size_t) P->RB + G) * 4;
   const size_t vsc_off  = row_off + ((size_t) 4 * P->R + G) * 4;
-  const size_t slot_bytes = ((vsc_off + (size_t) cm->M * 4) + 255) / 256 * 256;
+  const size_t acc_off  = ((vsc_off + (size_t) cm->M * 4) + 255) / 256 * 256;   /* global accumulators: rows j, j+1 */
+  const size_t slot_bytes = ((acc_off + (gacc ? (size_t) acc_n * 8 : 0)) + 255) / 256 * 256;
   size_t mem_free = 0, mem_total = 0;
   if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
   const size_t scratch_budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
@@ -632,6 +667,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.scratch = (char *) D->d_scratch.ptr;
   A.slot_bytes = (int64_t) slot_bytes;
   A.deck_off = (int64_t) deck_off; A.row_off = (int64_t) row_off; A.vsc_off = (int64_t) vsc_off;
+  A.acc_off = (int64_t) acc_off;
   A.acc_n = acc_n;
   A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB;
   A.i0 = i0; A.j0 = j0;
@@ -666,8 +702,9 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
 
   dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
   if (getenv("INFX_VERBOSE"))
-    fprintf(stderr, "[infx] %s: L %lld W %d chunk %lld nchunks %d nslots %d wpb %d blocks/CU %d slot %zu B table %s\n", fname,
-            (long long) L, W, (long long) chunk, nchunks, nslots, wpb, occ, slot_bytes, tbl_lds ? "lds" : "l2");
+    fprintf(stderr, "[infx] %s: L %lld W %d chunk %lld nchunks %d nslots %d wpb %d blocks/CU %d slot %zu B table %s begin-acc %s%s\n", fname,
+            (long long) L, W, (long long) chunk, nchunks, nslots, wpb, occ, slot_bytes, tbl_lds ? "lds" : "l2", gacc ? "global" : "lds",
+            wide ? " wide" : "");
   unsigned long long ncand = 0;
   float kernel_ms_total = 0;
   int launches = 0;
@@ -675,7 +712,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     if (hipMemsetAsync(A.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
     hipEventRecord(g_ev0, g_stream);
     if (upload(D->d_args, &A, sizeof(A)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
-    hipError_t e = infx_launch_std(semi, A.want_vsc, tbl_lds, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_n);
+    hipError_t e = infx_launch_std(semi, A.want_vsc, tbl_lds, gacc, wide, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_n);
     hipEventRecord(g_ev1, g_stream);
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, A.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||

@@ -74,12 +74,13 @@ struct ScanArgs {
   char          *scratch;
   int64_t        slot_bytes;
   int64_t        deck_off, row_off, vsc_off;   /* byte offsets inside a slot */
+  int64_t        acc_off;     /* local-begin accumulators in the slot (global-accumulator kernels) */
   int            M, W, R, RB;
   int64_t        i0, j0;
   int64_t        chunk_len;
   const int64_t *chunk_tab; /* null: chunk c covers i0 + c*chunk_len ..; else [3c..3c+2] = first row, last row, window start */
   int            nchunks, nslots;
-  int            acc_n;       /* LDS local-begin accumulators per wavefront */
+  int            acc_n;       /* local-begin accumulators per row and wavefront */
   /* root state */
   int            root_dn, root_dx;
   int            root_cnum;
@@ -142,6 +143,7 @@ struct TrPen {
 struct TrWindow {
   int64_t i0, j0;
   int64_t dsq_off;     /* device dsq + dsq_off, indexed by absolute j */
+  int32_t id, pad;     /* the caller's window index (windows run longest first) */
 };
 
 struct TrCand {
@@ -159,8 +161,12 @@ struct TrScanArgs {
   const void    *lsum;
   const TrPen   *pen;
   int            npen;
-  const TrWindow *win;
+  const int     *pen_blk;    /* [nblk + 1] offsets, then per 64-cell root block b the
+                                indices (ascending) of the begins whose band reaches d = 64b + 1 */
+  int            pen_nblk;
+  const TrWindow *win;       /* longest first */
   int            nwin;
+  unsigned      *next_win;   /* work counter: wavefronts take windows in order */
   char          *scratch;
   int64_t        slot_bytes;
   int            nslots;

@@ -109,9 +109,9 @@ __device__ __forceinline__ int shr1(int first, int x) { return __builtin_amdgcn_
 /* ------------------------------------------------------------------------ */
 /* semirings                                                                 */
 
-template <int S> struct SR;
+template <int S> struct SRB;
 
-template <> struct SR<SEMI_CYK> {
+template <> struct SRB<SEMI_CYK> {
   typedef float T;
   static __device__ __forceinline__ T imp() { return IMPF; }
   static __device__ __forceinline__ T neg_sentinel() { return -INFINITY; }
@@ -127,9 +127,12 @@ template <> struct SR<SEMI_CYK> {
     return ok ? __fadd_rn(__fmul_rn(__uint_as_float(el), (float) x), __uint_as_float(endsc)) : IMPF;
   }
   static __device__ __forceinline__ bool same(T a, T b) { return __float_as_uint(a) == __float_as_uint(b); }
+  /* order-preserving int key of a non-NaN float (an involution) */
+  static __device__ __forceinline__ int key(T x) { const int b = __float_as_int(x); return b ^ ((b >> 31) & 0x7fffffff); }
+  static __device__ __forceinline__ T unkey(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
 };
 
-template <> struct SR<SEMI_FINSIDE> {
+template <> struct SRB<SEMI_FINSIDE> {
   typedef float T;
   static __device__ __forceinline__ T imp() { return IMPF; }
   static __device__ __forceinline__ T neg_sentinel() { return -INFINITY; }
@@ -156,9 +159,12 @@ template <> struct SR<SEMI_FINSIDE> {
     return ok ? __fadd_rn(__fmul_rn(__uint_as_float(el), (float) x), __uint_as_float(endsc)) : IMPF;
   }
   static __device__ __forceinline__ bool same(T a, T b) { return __float_as_uint(a) == __float_as_uint(b); }
+  /* order-preserving int key of a non-NaN float (an involution) */
+  static __device__ __forceinline__ int key(T x) { const int b = __float_as_int(x); return b ^ ((b >> 31) & 0x7fffffff); }
+  static __device__ __forceinline__ T unkey(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
 };
 
-template <> struct SR<SEMI_IINSIDE> {
+template <> struct SRB<SEMI_IINSIDE> {
   typedef int T;
   static __device__ __forceinline__ T imp() { return -INFTY_I; }
   static __device__ __forceinline__ T neg_sentinel() { return INT_MIN; }
@@ -183,6 +189,16 @@ template <> struct SR<SEMI_IINSIDE> {
     return ok ? wrap_add(wrap_mul((int) el, x), (int) endsc) : -INFTY_I;
   }
   static __device__ __forceinline__ bool same(T a, T b) { return a == b; }
+  static __device__ __forceinline__ int key(T x) { return x; }
+  static __device__ __forceinline__ T unkey(int k) { return k; }
+};
+
+/* S = semiring | SEMI_GACC: the local-begin accumulators live in the
+ * wavefront's global scratch slot (integer-keyed atomic max at L2) instead
+ * of LDS, for root bands too wide for LDS at full occupancy */
+#define SEMI_GACC 8
+template <int S> struct SR : SRB<S & 7> {
+  static const bool GACC = (S & SEMI_GACC) != 0;
 };
 
 /* Combine child terms c[0..CN-1] and the init term in the reference's
@@ -249,9 +265,15 @@ template <int S>
 __device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bool on, int d, typename SR<S>::T sc, typename SR<S>::T bsc)
 {
   typedef SR<S> R;
-  /* LDS max without return (ds_max_f32 / ds_max_i32): no LDS read to wait
-   * for; lanes hold distinct d, and no value is NaN or -0.0 */
-  if (on) __hip_atomic_fetch_max(&acc[d - rdn], R::add(sc, bsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  /* max without return: no read to wait for; lanes hold distinct d, and no
+   * value is NaN.  LDS: ds_max_f32 / ds_max_i32.  Global: global_atomic_smax
+   * on order-preserving int keys (there is no global float max). */
+  if (R::GACC) {
+    int *a = (int *) ((char *) acc + (uint32_t) ((d - rdn) * 4));   /* SGPR base + 32-bit offset */
+    if (on) __hip_atomic_fetch_max(a, R::key(R::add(sc, bsc)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    if (on) __hip_atomic_fetch_max(&acc[d - rdn], R::add(sc, bsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
 }
 
 /* NB consecutive 64-cell blocks of a non-B, non-IL state, rows j (0) and
@@ -339,11 +361,13 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
 /* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
  * [dn, dx0] and [dn, dx1] (dx1 < dn: no row 1).  BEGL decks combine init, c0,
  * c1, ... in child order (src/cm_dpsearch.c:430-444); other states use
- * combine_children.  WIDE (Inside with the LDS table) takes two blocks at a
- * time where the band has them: its log-sum chains wait on LDS, and it runs
- * at 4 waves per SIMD, so the registers are free.  Returns the max score over the rows
+ * combine_children.  NBW > 1 takes up to NBW blocks at a time where the band
+ * has them (more loads in flight per wavefront, more registers): Inside with
+ * the LDS table, whose log-sum chains wait on LDS and which runs at 4 waves
+ * per SIMD anyway, and the wide-band kernels of models whose targets cannot
+ * fill the chip with wavefronts.  Returns the max score over the rows
  * flagged in vs. */
-template <int S, bool VSC, int CN, bool BEGL, bool WIDE>
+template <int S, bool VSC, int CN, bool BEGL, int NBW>
 __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
                                             const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
                                             typename SR<S>::T *out0, typename SR<S>::T *out1,
@@ -358,7 +382,11 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
   const int dxm = max(dx0, dx1);
   float vmax = IMPF;
   int d0 = dn;
-  if (WIDE)
+  if (NBW >= 4)
+    for (; d0 + 192 <= dxm; d0 += 256)
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 4>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
+  if (NBW >= 2)
     for (; d0 + 64 <= dxm; d0 += 128)
       vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
                                                          hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
@@ -550,9 +578,11 @@ __device__ __forceinline__ int begin_argmax(const DState *stp, int nst, const ty
   return br;
 }
 
-template <int S, bool VSC, bool TBL_LDS>
+template <int S, bool VSC, bool TBL_LDS, int NBC = 1>
 __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 {
+  /* blocks per step of the generic states (gen_state2) */
+  constexpr int NBW = ((S & 7) != SEMI_CYK && TBL_LDS) ? (NBC > 2 ? NBC : 2) : NBC;
   typedef SR<S> R;
   typedef typename R::T T;
   extern __shared__ __align__(16) char lds_raw[];
@@ -565,7 +595,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   /* log-sum table (Inside): staged in LDS, or read through L1/L2 */
   const T *tbl = (const T *) fresh(ap)->lsum;
   int lds_off = 0;
-  if (S != SEMI_CYK && TBL_LDS) {
+  if ((S & 7) != SEMI_CYK && TBL_LDS) {
     T *lt = (T *) lds_raw;
     const T *gt = (const T *) fresh(ap)->lsum;
     for (int i = threadIdx.x; i < INFX_LOGSUM_PAD; i += blockDim.x) lt[i] = gt[i];
@@ -576,10 +606,11 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   if (slot >= fresh(ap)->nslots) return;
 
   const int acc_n = fresh(ap)->acc_n;
-  T *acc0 = (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);   /* local-begin maxima of rows j, j+1 */
+  char *sbase = fresh(ap)->scratch + (size_t) slot * fresh(ap)->slot_bytes;
+  /* local-begin maxima of rows j, j+1 (global: int keys) */
+  T *acc0 = R::GACC ? (T *) (sbase + fresh(ap)->acc_off) : (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);
   T *acc1 = acc0 + acc_n;
 
-  char *sbase = fresh(ap)->scratch + (size_t) slot * fresh(ap)->slot_bytes;
   T   *deck = (T *) (sbase + fresh(ap)->deck_off);
   T   *rows = (T *) (sbase + fresh(ap)->row_off);
   float *vscs = (float *) (sbase + fresh(ap)->vsc_off);
@@ -634,8 +665,10 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
       const uint8_t *dsq_b0 = dsq_j - dsq_w;   /* left residue of cell d of row j0: dsq_b0[dsq_w - d] */
 
       if (local_begin) {
-        for (int d = rdn + lane; d <= rdx0; d += 64) acc0[d - rdn] = R::neg_sentinel();
-        for (int d = rdn + lane; d <= rdx1; d += 64) acc1[d - rdn] = R::neg_sentinel();
+        const T neg = R::GACC ? R::fromb(R::key(R::neg_sentinel())) : R::neg_sentinel();
+        for (int d = rdn + lane; d <= rdx0; d += 64) acc0[d - rdn] = neg;
+        for (int d = rdn + lane; d <= rdx1; d += 64) acc1[d - rdn] = neg;
+        if (R::GACC) __builtin_amdgcn_s_waitcnt(0);   /* the clears reach L2 before any atomic */
       }
 
       for (int i = 0; i < nst; i++) {
@@ -673,7 +706,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           const bool seq = (hdr >> 14) & 1;
           const int gx1 = seq ? dn - 1 : dx1;
           switch (cnum) {
-#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
           case 2:  GEN(2); break;
           case 3:  GEN(3); break;
           case 4:  GEN(4); break;
@@ -685,7 +718,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
             wave_sync();
             float vm1;
             switch (cnum) {
-#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
             case 2:  GEN(2); break;
             case 3:  GEN(3); break;
             case 4:  GEN(4); break;
@@ -710,7 +743,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           T *out0 = deck + (jp0 & rmask) * rstride;
           T *out1 = deck + (jp1 & rmask) * rstride;
           switch (cnum) {
-#define BGL(CN) vmax = gen_state2<S, VSC, CN, true, (S != SEMI_CYK && TBL_LDS)>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
+#define BGL(CN) vmax = gen_state2<S, VSC, CN, true, NBW>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
           case 1:  BGL(1); break;
           case 2:  BGL(2); break;
           case 3:  BGL(3); break;
@@ -731,6 +764,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         wave_sync();
       } /* states */
 
+      if (R::GACC && local_begin) __builtin_amdgcn_s_waitcnt(0);   /* every atomic done before the root reads */
       /* ROOT_S (src/cm_dpsearch.c:562-623) and hit candidates, row by row */
       for (int t = 0; t < (two ? 2 : 1); t++) {
         const int64_t j = j0 + t;
@@ -754,7 +788,9 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
             for (int k = 1; k < root_cnum; k++)
               al = R::op(al, R::add(ldu(cur, (uint32_t) (aq->root_child_base[k] + d)), R::bits(aq->root_tsc[k])), tbl);
             if (local_begin) {
-              const T acv = acc[d - rdn];
+              const T acv = R::GACC ? R::unkey(__hip_atomic_load((int *) ((char *) acc + (uint32_t) ((d - rdn) * 4)),
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                                    : acc[d - rdn];
               if (acv > al) { al = acv; lb = true; }
             }
             bs = R::score(al);
@@ -814,9 +850,15 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 /* Kernels.  CYK and Inside with the log-sum table read from L1/L2 are
  * register-allocated for 8 wavefronts per SIMD; Inside with the table in LDS
  * is limited to one 16-wavefront workgroup per CU by the table itself. */
-template <bool VSC>
+template <bool VSC, int G = 0>
 __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_cyk_kernel(const ScanArgs *a)
-{ std_scan_body<SEMI_CYK, VSC, false>(a); }
+{ std_scan_body<SEMI_CYK | G, VSC, false>(a); }
+
+/* wide-band CYK: four blocks in flight per step, for models whose targets
+ * give fewer chunks (of at least 2W residues) than the chip holds wavefronts */
+template <bool VSC, int G = 0>
+__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(4))) std_cyk_wide_kernel(const ScanArgs *a)
+{ std_scan_body<SEMI_CYK | G, VSC, false, 4>(a); }
 
 template <int S, bool VSC>
 __global__ void __launch_bounds__(64 * WPB_MAX) std_inside_lds_kernel(const ScanArgs *a)
@@ -829,8 +871,20 @@ __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_
 /* ------------------------------------------------------------------------ */
 /* launch stubs for scan_host.cpp                                            */
 
-static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds)
+static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds, int gacc, int wide)
 {
+  if (wide) {   /* CYK only */
+    if (semi != SEMI_CYK) return NULL;
+    if (gacc) return want_vsc ? (const void *) std_cyk_wide_kernel<true, SEMI_GACC> : (const void *) std_cyk_wide_kernel<false, SEMI_GACC>;
+    return want_vsc ? (const void *) std_cyk_wide_kernel<true> : (const void *) std_cyk_wide_kernel<false>;
+  }
+  if (gacc) {   /* global accumulators: CYK, and Inside with the table in LDS */
+    if (semi == SEMI_CYK) return want_vsc ? (const void *) std_cyk_kernel<true, SEMI_GACC> : (const void *) std_cyk_kernel<false, SEMI_GACC>;
+    if (!tbl_lds) return NULL;
+    if (semi == SEMI_FINSIDE) return want_vsc ? (const void *) std_inside_lds_kernel<SEMI_FINSIDE | SEMI_GACC, true> : (const void *) std_inside_lds_kernel<SEMI_FINSIDE | SEMI_GACC, false>;
+    if (semi == SEMI_IINSIDE) return want_vsc ? (const void *) std_inside_lds_kernel<SEMI_IINSIDE | SEMI_GACC, true> : (const void *) std_inside_lds_kernel<SEMI_IINSIDE | SEMI_GACC, false>;
+    return NULL;
+  }
   if (semi == SEMI_CYK) return want_vsc ? (const void *) std_cyk_kernel<true> : (const void *) std_cyk_kernel<false>;
   if (semi == SEMI_FINSIDE) {
     if (tbl_lds) return want_vsc ? (const void *) std_inside_lds_kernel<SEMI_FINSIDE, true> : (const void *) std_inside_lds_kernel<SEMI_FINSIDE, false>;
@@ -843,30 +897,31 @@ static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds)
   return NULL;
 }
 
-/* LDS bytes of a workgroup of `wpb` wavefronts */
-extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n)
+/* LDS bytes of a workgroup of `wpb` wavefronts (gacc: accumulators in the
+ * global slot, LDS holds only the Inside log-sum table) */
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc)
 {
-  return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) + (size_t) wpb * acc_n * 8;   /* two rows of 4-byte maxima */
+  return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) + (gacc ? 0 : (size_t) wpb * acc_n * 8);   /* two rows of 4-byte maxima */
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
-extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n)
+extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide)
 {
-  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds);
+  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds, gacc, wide);
   if (!f || wpb < 1 || wpb > WPB_MAX) return 0;
-  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, wpb, acc_n);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, wpb, acc_n, gacc);
   if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   return nb;
 }
 
-extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, dim3 grid, dim3 block, hipStream_t stream,
-                                      const ScanArgs *a, int acc_n)
+extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int gacc, int wide, dim3 grid, dim3 block,
+                                      hipStream_t stream, const ScanArgs *a, int acc_n)
 {
-  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds);
+  const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds, gacc, wide);
   if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
-  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n, gacc);
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
   void *args[] = { (void *) &a };

@@ -35,6 +35,18 @@ __device__ __forceinline__ void wave_sync_t()
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+/* read-only tables through the constant address space: uniform indices
+ * become scalar loads (one s_load per field group, cached in the K$) */
+typedef const __attribute__((address_space(4))) TState CTState;
+typedef const __attribute__((address_space(4))) TrPen CTrPen;
+typedef const __attribute__((address_space(4))) int CInt;
+__device__ __forceinline__ TrPen load_pen(const TrPen *base, int i)
+{
+  CTrPen *q = (CTrPen *) base + i;
+  TrPen p;
+  p.y = q->y; p.base = q->base; p.tbase = q->tbase; p.dx = q->dx; p.is_b = q->is_b; p.pen = q->pen;
+  return p;
+}
 __device__ __forceinline__ int wadd(int a, int b) { return (int) ((unsigned) a + (unsigned) b); }
 __device__ __forceinline__ int wmul(int a, int b) { return (int) ((unsigned) a * (unsigned) b); }
 /* x86 cvttss2si: truncation toward zero, out of range or NaN -> INT_MIN */
@@ -126,8 +138,16 @@ __global__ void __launch_bounds__(256) tr_scan_kernel(TrScanArgs a)
   Cc *H  = RB + (size_t) W1 * a.RB;           /* root history [4 modes][2][W+1] */
   float *vscs = (float *) (H + 8 * (size_t) W1);
 
-  for (int wdx = slot; wdx < a.nwin; wdx += a.nslots) {
-    const TrWindow win = a.win[wdx];
+  for (;;) {
+    /* the next window, longest first; every wavefront leaves once the
+     * counter has passed the last window.  All lanes take part in the add
+     * (lane 0 adds 1, the others 0) and the index is lane 0's result read
+     * into a scalar: with no lane-dependent branch around the atomic the
+     * index, and the loop exit, are wave-uniform by construction. */
+    const int nxt = __builtin_amdgcn_readlane((int) atomicAdd(a.next_win, lane == 0 ? 1u : 0u), 0);
+    if (nxt >= a.nwin) break;
+    const TrWindow win = a.win[nxt];
+    const int wdx = win.id;
     const uint8_t *dsq = a.dsq + win.dsq_off;   /* dsq[j] for j in i0..j0 */
     const int64_t i0 = win.i0, j0 = win.j0;
     const int Lw = (int) (j0 - i0 + 1);
@@ -164,7 +184,7 @@ __global__ void __launch_bounds__(256) tr_scan_kernel(TrScanArgs a)
       const int xj = dsq[j];
 
       for (int v = a.M - 1; v >= 1; v--) {
-        const TState *s = &a.st[v];
+        CTState *s = (CTState *) (a.st + v);   /* scalar loads: the descriptor is read-only */
         const int kind = s->kind;
         if (kind == KIND_SKIP) continue;
         const int dx = min(min(jp, We), s->dx);
@@ -421,14 +441,17 @@ __global__ void __launch_bounds__(256) tr_scan_kernel(TrScanArgs a)
       const int rdx = min(min(jp, We), a.root_dx);
       Cc *HJ = H + (size_t) (0 * 2 + cur) * W1, *HL = H + (size_t) (1 * 2 + cur) * W1;
       Cc *HR = H + (size_t) (2 * 2 + cur) * W1, *HT = H + (size_t) (3 * 2 + cur) * W1;
-      for (int d0 = 1; d0 <= rdx; d0 += 64) {
+      for (int d0 = 1, blk = 0; d0 <= rdx; d0 += 64, blk++) {
         const int d = d0 + lane;
         const bool act = d <= rdx;
         float bsc = IMPF; int bmode = TRMODE_UNKNOWN_K, br = 0;
         if (act) {
           Cc hj = HJ[d], hl = HL[d], hr = HR[d], ht = HT[d];
-          for (int pi = 0; pi < a.npen; pi++) {
-            const TrPen p = a.pen[pi];
+          /* the begins whose band reaches this block, ascending */
+          const CInt *pb = (const CInt *) a.pen_blk;
+          const int t0 = pb[blk], t1 = pb[blk + 1];
+          for (int t = t0; t < t1; t++) {
+            const TrPen p = load_pen(a.pen, pb[a.pen_nblk + 1 + t]);
             if (d > min(rdx, min(min(jp, We), p.dx))) continue;
             const Cc pen = R::bits(p.pen);
             Cc sc = R::add(Jc[p.base + d], pen);

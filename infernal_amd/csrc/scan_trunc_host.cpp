@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <chrono>
+#include <algorithm>
 #include <vector>
 
 #include "cm_internal.h"
@@ -26,9 +27,10 @@ struct TrProfile {
   std::vector<TState> st;
   std::vector<uint32_t> esc, rtmpl, dtmpl;
   std::vector<TrPen> pen;
+  std::vector<int> pblk;     /* per root block: offsets, then begin indices */
   int R = 0, RB = 0, RT = 0, root_dx = 0;
   uint32_t el = 0;
-  DevBuf d_st, d_esc, d_rtmpl, d_dtmpl, d_pen;
+  DevBuf d_st, d_esc, d_rtmpl, d_dtmpl, d_pen, d_pblk;
   bool uploaded = false;
 };
 
@@ -36,7 +38,7 @@ struct TrDev {
   std::vector<TrProfile *> prof;
   DevBuf d_dsq, d_scratch, d_cand, d_ncand, d_win, d_wout, d_lsum;
   ~TrDev() {
-    for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); p->d_pen.release(); delete p; }
+    for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); p->d_pen.release(); p->d_pblk.release(); delete p; }
     d_dsq.release(); d_scratch.release(); d_cand.release(); d_ncand.release(); d_win.release(); d_wout.release(); d_lsum.release();
   }
 };
@@ -197,6 +199,21 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
     P.pen.push_back(p);
   }
   P.root_dx = dx[0];
+  /* the root's truncated begins per 64-cell block of d: a begin whose band
+   * ends before the block changes none of its cells, so the block visits
+   * only the others, still in ascending state order */
+  {
+    const int nblk = (W + 63) / 64 + 1;
+    P.pblk.assign(nblk + 1, 0);
+    std::vector<int> idx;
+    for (int b = 0; b < nblk; b++) {
+      P.pblk[b] = (int) idx.size();
+      for (int k = 0; k < (int) P.pen.size(); k++)
+        if (P.pen[k].dx >= 64 * b + 1) idx.push_back(k);
+    }
+    P.pblk[nblk] = (int) idx.size();
+    P.pblk.insert(P.pblk.end(), idx.begin(), idx.end());
+  }
   P.el = (semi == TSEMI_IINSIDE) ? (uint32_t) cm->iel_selfsc : fb(cm->el_selfsc);
   if (P.esc.empty()) P.esc.push_back(0);
   if (P.pen.empty()) { TrPen z; memset(&z, 0, sizeof(z)); P.pen.push_back(z); P.pen.pop_back(); }
@@ -259,33 +276,42 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
               (fresh.pen.empty() || memcmp(P->pen.data(), fresh.pen.data(), fresh.pen.size() * sizeof(TrPen)) == 0) &&
               P->root_dx == fresh.root_dx && P->el == fresh.el && P->local == fresh.local;
   if (!same) {
-    P->st = fresh.st; P->esc = fresh.esc; P->rtmpl = fresh.rtmpl; P->dtmpl = fresh.dtmpl; P->pen = fresh.pen;
+    P->st = fresh.st; P->esc = fresh.esc; P->rtmpl = fresh.rtmpl; P->dtmpl = fresh.dtmpl; P->pen = fresh.pen; P->pblk = fresh.pblk;
     P->R = fresh.R; P->RB = fresh.RB; P->RT = fresh.RT; P->root_dx = fresh.root_dx; P->el = fresh.el;
     P->qdbidx = qdbidx; P->semi = semi; P->pty = pty; P->local = fresh.local;
     if (upload_buf(P->d_st, P->st.data(), P->st.size() * sizeof(TState), stream) != eslOK ||
         upload_buf(P->d_esc, P->esc.data(), P->esc.size() * 4, stream) != eslOK ||
         upload_buf(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4, stream) != eslOK ||
         upload_buf(P->d_dtmpl, P->dtmpl.data(), P->dtmpl.size() * 4, stream) != eslOK ||
-        upload_buf(P->d_pen, P->pen.data(), P->pen.size() * sizeof(TrPen), stream) != eslOK)
+        upload_buf(P->d_pen, P->pen.data(), P->pen.size() * sizeof(TrPen), stream) != eslOK ||
+        upload_buf(P->d_pblk, P->pblk.data(), P->pblk.size() * sizeof(int), stream) != eslOK)
       return infx_fail(eslESYS, errbuf, "%s: device upload failed", fname);
     P->uploaded = true;
   }
   if (semi == TSEMI_IINSIDE && D->d_lsum.cap == 0)
     if (upload_buf(D->d_lsum, infx_ilogsum_table(), LOGSUM_TBL * 4, stream) != eslOK) return infx_fail(eslESYS, errbuf, "upload failed");
 
-  /* windows: residues copied back to back */
+  /* windows: residues copied back to back, the longest first (each
+   * wavefront scans one window at a time, in this order) */
+  std::vector<int> order(nwin);
+  for (int w = 0; w < nwin; w++) order[w] = w;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wj0[a] - wi0[a] > wj0[b] - wi0[b]; });
   std::vector<TrWindow> hw(nwin);
   std::vector<uint8_t> packed;
   packed.reserve((size_t) total);
-  for (int w = 0; w < nwin; w++) {
+  for (int k = 0; k < nwin; k++) {
+    const int w = order[k];
     int64_t off = (int64_t) packed.size();
     packed.insert(packed.end(), dsq + wi0[w], dsq + wj0[w] + 1);
-    hw[w].i0 = wi0[w]; hw[w].j0 = wj0[w]; hw[w].dsq_off = off - wi0[w];
+    hw[k].i0 = wi0[w]; hw[k].j0 = wj0[w]; hw[k].dsq_off = off - wi0[w]; hw[k].id = w; hw[k].pad = 0;
   }
   const int W = trsmx->W;
   const int wpb = infx_wpb_override() ? ESL_MIN(infx_wpb_override(), 4) : 4;
   const size_t slot_bytes = (((size_t) 6 * P->R + 2 * (size_t) P->RT + 3 * (size_t) (W + 1) * P->RB + 8 * (size_t) (W + 1) + cm->M) * 4 + 255) / 256 * 256;
-  const size_t budget = (size_t) 24 << 30;
+  size_t mem_free = 0, mem_total = 0;
+  if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
+  const size_t budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
+                        + D->d_scratch.cap;
   int nslots = (int) ESL_MIN((int64_t) nwin, (int64_t) (budget / slot_bytes));
   nslots = ESL_MIN(nslots, infx_num_cu() * 16);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
@@ -293,7 +319,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   if (upload_buf(D->d_dsq, packed.data(), packed.size(), stream) != eslOK ||
       upload_buf(D->d_win, hw.data(), hw.size() * sizeof(TrWindow), stream) != eslOK ||
       D->d_scratch.reserve(slot_bytes * nslots) != eslOK || D->d_wout.reserve(wout_bytes) != eslOK ||
-      D->d_ncand.reserve(64) != eslOK)
+      D->d_ncand.reserve(128) != eslOK)
     return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, (int64_t) (D->d_cand.cap / sizeof(TrCand)));
   if (D->d_cand.reserve((size_t) cand_cap * sizeof(TrCand)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
@@ -304,6 +330,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   A.st = (const TState *) P->d_st.ptr; A.esc = P->d_esc.ptr; A.dsq = (const uint8_t *) D->d_dsq.ptr;
   A.row_tmpl = P->d_rtmpl.ptr; A.deck_tmpl = P->d_dtmpl.ptr; A.lsum = D->d_lsum.ptr;
   A.pen = (const TrPen *) P->d_pen.ptr; A.npen = (int) P->pen.size();
+  A.pen_blk = (const int *) P->d_pblk.ptr; A.pen_nblk = (W + 63) / 64 + 1;
   A.win = (const TrWindow *) D->d_win.ptr; A.nwin = nwin;
   A.scratch = (char *) D->d_scratch.ptr; A.slot_bytes = (int64_t) slot_bytes; A.nslots = nslots;
   A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB; A.RT = P->RT; A.root_dx = P->root_dx;
@@ -314,6 +341,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   A.cutoff = want_hits ? cutoff : INFINITY;
   A.env_cutoff = env_cutoff;
   A.cand = (TrCand *) D->d_cand.ptr; A.ncand = (unsigned long long *) D->d_ncand.ptr; A.cand_cap = cand_cap;
+  A.next_win = (unsigned *) ((char *) D->d_ncand.ptr + 64);
   char *wo = (char *) D->d_wout.ptr;
   A.win_best = (float *) wo;
   A.win_mode = (int *) (wo + (size_t) nwin * 4);
@@ -327,7 +355,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   hipEvent_t e0, e1;
   hipEventCreate(&e0); hipEventCreate(&e1);
   for (int attempt = 0; attempt < 3; attempt++) {
-    if (hipMemsetAsync(A.ncand, 0, 8, stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
+    if (hipMemsetAsync(A.ncand, 0, 128, stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
     hipEventRecord(e0, stream);
     hipError_t e = infx_launch_tr(semi, want_vsc, grid, block, stream, &A);
     hipEventRecord(e1, stream);

@@ -135,3 +135,30 @@ def plant(background, frags, seed):
         starts.append(pos + 1)
         pos += len(f)
     return seq, starts
+
+
+def truncated_windows(cm, n, seed, lmin=200, lmax=2000, min_overlap=100, pool=100):
+    """n windows of uniform length in [lmin, lmax], half holding an emitted
+    homolog clipped by the window at a random offset (so it is cut at the
+    window's 5' end, its 3' end, both or neither), half uniform random ACGU
+    decoys; concatenated into one target.  The homologs are drawn from a pool
+    of `pool` emitted sequences (emission is the slow part).  Returns
+    (codes, windows, is_hom) with 1-based inclusive (i, j) window bounds."""
+    rng = np.random.default_rng(seed)
+    seqs = [emit(cm, rng) for _ in range(max(1, min(pool, (n + 1) // 2)))]
+    parts, wins, hom = [], [], []
+    pos = 1
+    for k in range(n):
+        L = int(rng.integers(lmin, lmax + 1))
+        w = rng.integers(0, 4, L).astype(np.uint8)
+        if k % 2 == 0:
+            s = seqs[int(rng.integers(0, len(seqs)))]
+            ov = min(min_overlap, len(s), L)
+            o = int(rng.integers(-(len(s) - ov), L - ov + 1))   # s starts at window offset o
+            a, b = max(0, -o), min(len(s), L - o)
+            w[o + a:o + b] = s[a:b]
+        parts.append(w)
+        wins.append((pos, pos + L - 1))
+        hom.append(k % 2 == 0)
+        pos += L
+    return np.concatenate(parts), wins, np.array(hom)

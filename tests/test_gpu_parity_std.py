@@ -63,6 +63,29 @@ def test_cyk_chunked_long():
         ia.set_launch_params(0, 0)
 
 
+def test_cyk_wide_kernel_chunked():
+    """The wide-band CYK kernel over many short chunks, a window start that
+    is not a chunk boundary, and short targets."""
+    os.environ["INFX_WIDE"] = "1"
+    ia.set_launch_params(chunk_len=500)
+    try:
+        cm = load_cm("tRNA.c.cm")
+        dsq = random_dsq(20000, seed=4)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 20000, 0.0, env=3.0, vsc=True)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 777, 19000, 0.0, env=3.0)
+        ia.set_launch_params(0, 0)
+        for L in (1, 7, 218, 219, 1000):
+            d2 = random_dsq(L, seed=200 + L)
+            _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, d2, 1, L, -50.0, vsc=True)
+        cm = load_cm("rnaseP-eubact.c.cm")
+        d3 = random_dsq(3000, seed=61)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, d3, 1, 3000, 0.0, vsc=True)
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 2, d3, 1, 3000, 0.0, null3=True, greedy=False)
+    finally:
+        os.environ.pop("INFX_WIDE", None)
+        ia.set_launch_params(0, 0)
+
+
 def test_cyk_null3_and_gamma():
     cm = load_cm("Vault.c.cm")
     dsq = random_dsq(4000, seed=9, gc=0.6)
@@ -130,12 +153,35 @@ def test_synthetic_models(name):
     _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 4000, 0.0, tol=FINSIDE_TOL)
 
 
-def test_synthetic_1500nt_cyk():
-    """The 1,500-nt model (W 1522): CYK over a target spanning several chunks."""
+@pytest.mark.parametrize("placement", ["auto", "lds"])
+def test_synthetic_1500nt_cyk(placement, monkeypatch):
+    """The 1,500-nt model (W 1522): CYK over a target spanning several chunks,
+    with the local-begin accumulators where the launch puts them (the global
+    slot, for this root band) and forced into LDS."""
+    if placement != "auto":
+        monkeypatch.setenv("INFX_BEGIN_ACC", placement)
     cm = load_cm("synth_nested1500.cm")
     dsq = _planted("synth_nested1500.cm", 5000, 2, seed=31)
     g, _ = _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 5000, 0.0, vsc=True)
     assert g["sc"] > 50.0
+
+
+@pytest.mark.parametrize("placement,wide", [("global", "0"), ("lds", "0"), ("global", "1"), ("lds", "1")])
+def test_begin_accumulator_placements(placement, wide, monkeypatch):
+    """Local-begin maxima in the wavefront's global slot (int-keyed atomic
+    max) or in LDS, and the wide-band CYK kernel (four blocks per step) or
+    the narrow one, give the same bit-exact results for all three scanners."""
+    monkeypatch.setenv("INFX_BEGIN_ACC", placement)
+    monkeypatch.setenv("INFX_WIDE", wide)
+    cm = load_cm("synth_multiloop350.cm")
+    dsq = _planted("synth_multiloop350.cm", 6000, 4, seed=23)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 6000, 0.0, env=5.0, vsc=True)
+    _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, 1, 4000, 0.0, tol=FINSIDE_TOL)
+    _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 2, dsq, 1, 4000, 0.0, vsc=True)
+    cm = load_cm("Cobalamin.c.cm")
+    dsq = random_dsq(2500, seed=57)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 1, 2500, -5.0, vsc=True, greedy=False)
+    _run(ia.FastIInsideScan, ob.ORACLE_IINSIDE, cm, 1, dsq, 1, 2500, 0.0, null3=True)
 
 
 @pytest.mark.parametrize("algo", ["cyk", "finside"])
