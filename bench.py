@@ -143,11 +143,12 @@ def trunc_leg(args, ia, synth, threads):
     path = os.path.join(ROOT, "tests", "golden", "cm", "synth_nested1500.cm")
     cm = ia.CM.read(path)
     cm.configure(local=True, trunc=True, scanmx=True, trscanmx=True)
-    codes, wins, hom = synth.truncated_windows(ia.CM.read(path), args.trunc_windows, seed=33, pool=50)
+    log("truncated leg: %d windows of 200-%d nt" % (args.trunc_windows, args.trunc_lmax))
+    codes, wins, hom = synth.truncated_windows(ia.CM.read(path), args.trunc_windows, seed=33, lmax=args.trunc_lmax, pool=50)
     dsq = ia.dsq_from_codes(codes)
     pidx = ia.PLI_PASS_5P_AND_3P_FORCE
     out = {"cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), cm.clen, cm.M, cm.W),
-           "windows": len(wins), "residues": int(len(codes)), "pass": "5P_AND_3P_FORCE",
+           "windows": len(wins), "window_nt": [200, args.trunc_lmax], "residues": int(len(codes)), "pass": "5P_AND_3P_FORCE",
            "note": "cells are banded (v, j, d) cells; each is computed in J, L and R mode (T too for bifurcations)"}
     # launches of --trunc-batch windows of similar length (one wavefront per
     # window: a launch lasts as long as its longest window)
@@ -191,6 +192,7 @@ def trunc_leg(args, ia, synth, threads):
         with cf.ThreadPoolExecutor(len(sample)) as ex:
             list(ex.map(one, sample))
         cw = time.perf_counter() - t0
+        log("truncated CPU sample: %d windows in %.1f s" % (len(sample), cw))
         cells = sum(_cells(cm, 1, int(w[0]), int(w[1]), truncated=True) for w in sample)
         t1 = time.perf_counter()
         one(sample[0])
@@ -308,9 +310,13 @@ def main():
     ap.add_argument("--gate-quantile", type=float, default=0.005, help="held-out hit quantile the threshold sits at")
     ap.add_argument("--cyk1500-mb", type=float, default=20.0,
                     help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
-    ap.add_argument("--trunc-windows", type=int, default=32,
-                    help="windows of the truncated-scan leg (1,500-nt model, 200-2,000 nt each); 0 = skip")
-    ap.add_argument("--trunc-batch", type=int, default=32, help="windows per truncated-scan launch")
+    ap.add_argument("--trunc-windows", type=int, default=0,
+                    help="windows of the truncated-scan leg (1,500-nt model, 200 to --trunc-lmax nt each; "
+                         "run in a child process); 0 = skip")
+    ap.add_argument("--trunc-lmax", type=int, default=1000,
+                    help="longest truncated-scan window (a window is one wavefront's serial scan: time grows with its cells)")
+    ap.add_argument("--trunc-batch", type=int, default=16, help="windows per truncated-scan launch")
+    ap.add_argument("--trunc-only", action="store_true", help="run only the truncated-scan leg and print its JSON")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -338,6 +344,9 @@ def main():
         sys.exit(1)
     if args.chunk:
         ia.set_launch_params(args.chunk, 0)
+    if args.trunc_only:
+        print(json.dumps(trunc_leg(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))))
+        return
 
     cm = ia.CM.read(args.cm)
     cm.configure(local=True)
@@ -409,6 +418,7 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         r = cpu_baseline(cm, codes, threads, args.cpu_sample, W)
+        log("CPU baseline: CYK %.1f s, FInside %.1f s in %d threads" % (r["cyk"][1], r["finside"][1], threads))
         cc = r["cyk"][0] + r["finside"][0]
         ct = r["cyk"][1] + r["finside"][1]
         cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
@@ -431,7 +441,23 @@ def main():
 
     trunc = None
     if args.trunc_windows > 0:
-        trunc = trunc_leg(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
+        # in a child process of its own: the leg's device buffers and host
+        # memory are released when it ends, and the JSON line above does not
+        # depend on it
+        import subprocess
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--trunc-only", "--trunc-windows", str(args.trunc_windows),
+               "--trunc-lmax", str(args.trunc_lmax), "--trunc-batch", str(args.trunc_batch)]
+        if args.no_cpu:
+            cmd.append("--no-cpu")
+        if args.cpu_threads:
+            cmd += ["--cpu-threads", str(args.cpu_threads)]
+        pr = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        lines = [x for x in pr.stdout.splitlines() if x.startswith("{")]
+        if pr.returncode == 0 and lines:
+            trunc = json.loads(lines[-1])
+        else:
+            trunc = {"error": "truncated leg exited with status %d" % pr.returncode}
+            log("truncated leg failed: status %d" % pr.returncode)
 
     out = {
         "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
