@@ -16,6 +16,13 @@ target's end.  Every access must fall inside its buffer:
   LDS       accumulator index [0, acc_n)
   emission  [0, nesc)
 
+The replay walks the 64-cell blocks of a state one at a time.  The kernels
+that take two or four blocks per step (Inside with the LDS table, the
+wide-band CYK kernel) visit the same blocks with the same per-block masks,
+which test_multi_block_steps_visit_the_same_blocks checks; the kernels with
+the local-begin maxima in the global slot use the same accumulator indices
+(row j at [0, acc_n), row j+1 at [acc_n, 2 acc_n) of the slot's region).
+
 This runs without a GPU, before any kernel change goes to one.
 """
 import ctypes as C
@@ -270,3 +277,28 @@ def test_checker_catches_planted_overreach():
     assert not chk.errors
     chk.dsq(np.array([64 + j0 - dn]), "planted")
     assert chk.errors
+
+
+def _blocks(dn, dxm, nbw):
+    """Block starts gen_state2 (scan_std.hip) visits with up to nbw blocks per step."""
+    out, d0 = [], dn
+    if nbw >= 4:
+        while d0 + 192 <= dxm:
+            out += [d0, d0 + 64, d0 + 128, d0 + 192]
+            d0 += 256
+    if nbw >= 2:
+        while d0 + 64 <= dxm:
+            out += [d0, d0 + 64]
+            d0 += 128
+    while d0 <= dxm:
+        out.append(d0)
+        d0 += 64
+    return out
+
+
+def test_multi_block_steps_visit_the_same_blocks():
+    for dn in (0, 1, 2, 5, 63, 64):
+        for dxm in range(dn - 1, dn + 700):
+            one = _blocks(dn, dxm, 1)
+            assert one == list(range(dn, dxm + 1, 64))
+            assert _blocks(dn, dxm, 2) == one and _blocks(dn, dxm, 4) == one, (dn, dxm)
