@@ -321,7 +321,11 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
       D->d_scratch.reserve(slot_bytes * nslots) != eslOK || D->d_wout.reserve(wout_bytes) != eslOK ||
       D->d_ncand.reserve(128) != eslOK)
     return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
-  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, (int64_t) (D->d_cand.cap / sizeof(TrCand)));
+  /* candidate buffer: eight per residue (a cutoff near 0 keeps many cells of
+   * each row), at most 8 GB; an overflow re-runs the launch once with the
+   * exact size */
+  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 8 * (int64_t) total, (int64_t) (((size_t) 8 << 30) / sizeof(TrCand))));
+  cand_cap = ESL_MAX(cand_cap, (int64_t) (D->d_cand.cap / sizeof(TrCand)));
   if (D->d_cand.reserve((size_t) cand_cap * sizeof(TrCand)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(TrCand));
 
