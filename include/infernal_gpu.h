@@ -414,6 +414,19 @@ int infx_FastScanWindows(int semi, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int 
                          const int64_t *wi, const int64_t *wj, int nwin, float cutoff, CM_TOPHITS *hitlist,
                          int do_null3, float *ret_sc);
 
+/* Several models in one launch: window w (wi[w]..wj[w] of dsq) is scanned
+ * with model wm[w] (cms[wm[w]], smxs[wm[w]]); each model's windows are sorted
+ * and disjoint, windows of different models may overlap.  Model m's hits
+ * (greedy or gamma as its search_opts say, no NULL3, cutoff cutoffs[m]) are
+ * appended to hitlists[m] and equal those of infx_FastScanWindows over its
+ * own windows; ret_sc[m] is its best score.  No counterpart in the
+ * reference: for callers scanning many models of uneven size (cmscan, a
+ * model library against one sequence set) in one launch instead of one per
+ * model. */
+int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX **smxs, char *errbuf, int qdbidx, ESL_DSQ *dsq,
+                       const int64_t *wi, const int64_t *wj, const int32_t *wm, int nwin, const float *cutoffs,
+                       CM_TOPHITS **hitlists, float *ret_sc);
+
 /* Override the work split (0 = automatic): residues per chunk and waves per
  * block.  Used by the autotuner and tests. */
 void infx_set_launch_params(int chunk_len, int waves_per_block);

@@ -333,6 +333,47 @@ def ScanWindows(algo, cm, qdbidx, dsq, windows, cutoff, do_null3=False, greedy=T
     return {"hits": hits, "sc": sc.value}
 
 
+def ScanBatch(algo, cms, qdbidx, dsq, windows, cutoffs, greedy=True):
+    """infx_FastScanBatch: several models' windows of one target in one
+    launch; windows are (i, j, model index), each model's sorted and
+    disjoint.  Returns per model {"hits": [...], "sc": best}, equal to
+    ScanWindows over that model's windows."""
+    semi = {"cyk": 0, "finside": 1, "iinside": 2}[algo]
+    n = len(cms)
+    for cm in cms:
+        c = cm.c
+        if semi:
+            c.search_opts |= CM_SEARCH_INSIDE
+        else:
+            c.search_opts &= ~CM_SEARCH_INSIDE
+        if greedy:
+            c.search_opts &= ~CM_SEARCH_CMNOTGREEDY
+        else:
+            c.search_opts |= CM_SEARCH_CMNOTGREEDY
+    dsq = np.ascontiguousarray(dsq, dtype=np.uint8)
+    wi = np.ascontiguousarray([w[0] for w in windows], dtype=np.int64)
+    wj = np.ascontiguousarray([w[1] for w in windows], dtype=np.int64)
+    wm = np.ascontiguousarray([w[2] for w in windows], dtype=np.int32)
+    cms_arr = (C.c_void_p * n)(*[C.cast(cm.ptr, C.c_void_p) for cm in cms])
+    smx_arr = (C.c_void_p * n)(*[C.cast(cm.c.smx, C.c_void_p) for cm in cms])
+    cut = np.ascontiguousarray(cutoffs, dtype=np.float32)
+    ths = [_lib.cm_tophits_Create() for _ in range(n)]
+    th_arr = (C.c_void_p * n)(*[C.cast(t, C.c_void_p) for t in ths])
+    sc = np.zeros(n, np.float32)
+    f = _lib.infx_FastScanBatch
+    f.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    eb = C.create_string_buffer(ERRBUF)
+    st = f(semi, n, cms_arr, smx_arr, eb, qdbidx, dsq.ctypes.data, wi.ctypes.data, wj.ctypes.data, wm.ctypes.data,
+           len(windows), cut.ctypes.data, th_arr, sc.ctypes.data)
+    out = [{"hits": _hits_list(t), "sc": float(sc[m])} for m, t in enumerate(ths)] if st == eslOK else None
+    for t in ths:
+        _lib.cm_tophits_Destroy(t)
+    if st != eslOK:
+        raise ScanError(st, eb.value.decode(errors="replace"))
+    return out
+
+
 def TrScanWindows(algo, cm, qdbidx, pass_idx, dsq, windows, cutoff, do_null3=False, env_cutoff=0.0, want_hits=True,
                   greedy=True):
     """infx_TrScanWindows: one truncated scan per window [(i, j), ...] of one

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "cm_internal.h"
@@ -19,6 +20,8 @@ extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int g
                                       hipStream_t stream, const ScanArgs *a, int acc_n);
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc);
 extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
+extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n);
+extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a, int acc_n);
 
 /* ------------------------------------------------------------------------ */
 /* scan matrices (src/cm_mx.c:5686-5836, :6391-6560)                          */
@@ -217,12 +220,12 @@ struct StdProfile {
 
 struct SmxDev {
   std::vector<StdProfile *> prof;
-  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args, d_ctab;
+  DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args, d_ctab, d_btab;
   void  *trdev = NULL;   /* truncated-scanner resources */
   ~SmxDev() {
     for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
     d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release(); d_ctab.release();
-    d_lsum_f.release(); d_lsum_i.release();
+    d_btab.release(); d_lsum_f.release(); d_lsum_i.release();
     infx_trdev_release(trdev);
   }
 };
@@ -859,6 +862,272 @@ extern "C" int infx_FastScanWindows(int semi, CM_t *cm, char *errbuf, CM_SCAN_MX
   if (nwin < 1 || wi == NULL || wj == NULL) return infx_fail(eslEINCOMPAT, errbuf, "infx_FastScanWindows: no windows");
   return std_scan(semi, names[semi], cm, errbuf, smx, qdbidx, dsq, wi, wj, nwin, cutoff, hitlist, do_null3, 0.f,
                   NULL, NULL, NULL, ret_sc);
+}
+
+/* Several models' windows of one target in one launch.  Window w belongs to
+ * model wm[w]; each model's windows are sorted and disjoint (windows of
+ * different models may overlap).  Every model's hits are those of
+ * infx_FastScanWindows over its own windows (greedy reporting, no NULL3),
+ * appended to hitlists[m]; ret_sc[m] is its best score.  Chunks of every
+ * model share one launch: the slot is sized for the largest model and each
+ * chunk reads its own model's parameters. */
+extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX **smxs, char *errbuf, int qdbidx, ESL_DSQ *dsq,
+                                  const int64_t *wi, const int64_t *wj, const int32_t *wm, int nwin, const float *cutoffs,
+                                  CM_TOPHITS **hitlists, float *ret_sc)
+{
+  static const char *names[3] = { "FastCYKScan", "FastFInsideScan", "FastIInsideScan" };
+  int status;
+  auto t_start = std::chrono::steady_clock::now();
+  infx_timing_reset();
+  if (semi < SEMI_CYK || semi > SEMI_IINSIDE) return infx_fail(eslEINVAL, errbuf, "infx_FastScanBatch: bad algorithm %d", semi);
+  const char *fname = names[semi];
+  if (nmodel < 1 || nmodel > 32767 || !cms || !smxs) return infx_fail(eslEINCOMPAT, errbuf, "infx_FastScanBatch: bad model count %d", nmodel);
+  if (nwin < 1 || !wi || !wj || !wm || !dsq) return infx_fail(eslEINCOMPAT, errbuf, "infx_FastScanBatch: no windows");
+  /* per-model contract checks, as the single-model scanners make them */
+  std::vector<int64_t> lo_m(nmodel, INT64_MAX), hi_m(nmodel, 0), lmax_m(nmodel, 0), last_j(nmodel, 0);
+  int64_t span_lo = INT64_MAX, span_hi = 0, Ltot = 0;
+  for (int w = 0; w < nwin; w++) {
+    const int m = wm[w];
+    if (m < 0 || m >= nmodel) return infx_fail(eslEINVAL, errbuf, "%s, window %d: model index %d out of range", fname, w, m);
+    if (wi[w] < 1 || wj[w] < wi[w]) return infx_fail(eslEINCOMPAT, errbuf, "%s, i0: %lld j0: %lldd\n", fname, (long long) wi[w], (long long) wj[w]);
+    if (last_j[m] && wi[w] <= last_j[m]) return infx_fail(eslEINCOMPAT, errbuf, "%s, windows of one model must be sorted and disjoint", fname);
+    last_j[m] = wj[w];
+    lo_m[m] = ESL_MIN(lo_m[m], wi[w]); hi_m[m] = ESL_MAX(hi_m[m], wj[w]);
+    lmax_m[m] = ESL_MAX(lmax_m[m], wj[w] - wi[w] + 1);
+    span_lo = ESL_MIN(span_lo, wi[w]); span_hi = ESL_MAX(span_hi, wj[w]);
+    Ltot += wj[w] - wi[w] + 1;
+  }
+  if (span_hi - span_lo + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
+  for (int64_t x = span_lo; x <= span_hi; x++)
+    if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
+  std::vector<int> Wm(nmodel, 0);
+  std::vector<StdProfile> fresh(nmodel);
+  for (int m = 0; m < nmodel; m++) {
+    CM_t *cm = cms[m]; CM_SCAN_MX *smx = smxs[m];
+    if (!cm || !smx)                    return infx_fail(eslEINCOMPAT, errbuf, "%s, model %d: cm or smx is NULL", fname, m);
+    if (!(cm->flags & CMH_BITS))        return infx_fail(eslEINCOMPAT, errbuf, "%s, CMH_BITS flag is not raised.\n", fname);
+    if (semi == SEMI_CYK && (cm->search_opts & CM_SEARCH_INSIDE))  return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag raised", fname);
+    if (semi != SEMI_CYK && !(cm->search_opts & CM_SEARCH_INSIDE)) return infx_fail(eslEINCOMPAT, errbuf, "%s, CM_SEARCH_INSIDE flag not raised", fname);
+    if (semi == SEMI_CYK && !smx->floats_valid) return infx_fail(eslEINCOMPAT, errbuf, "%s, smx->floats_valid if FALSE", fname);
+    if (semi != SEMI_CYK && !smx->ints_valid)   return infx_fail(eslEINCOMPAT, errbuf, "%s, smx->ints_valid if FALSE", fname);
+    if (cm->qdbinfo == NULL)            return infx_fail(eslEINCOMPAT, errbuf, "%s, cm->qdbinfo == NULL\n", fname);
+    if (qdbidx < 0 || qdbidx > 2)       return infx_fail(eslEINCOMPAT, errbuf, "%s, qdbidx is invalid", fname);
+    if (lmax_m[m] == 0) continue;       /* a model without windows */
+    Wm[m] = (smx->W > lmax_m[m]) ? (int) lmax_m[m] : smx->W;
+    if ((status = build_std_profile(cm, smx, qdbidx, semi, Wm[m], fresh[m], errbuf)) != eslOK) return status;
+  }
+  if ((status = infx_dev_init(errbuf)) != eslOK) return status;
+
+  /* each model's profile on the device (cached with its scan matrix) */
+  std::vector<StdProfile *> prof(nmodel, (StdProfile *) NULL);
+  for (int m = 0; m < nmodel; m++) {
+    if (lmax_m[m] == 0) continue;
+    CM_SCAN_MX *smx = smxs[m];
+    if (!smx->dev) smx->dev = new SmxDev();
+    SmxDev *Dm = (SmxDev *) smx->dev;
+    StdProfile *P = NULL;
+    for (auto *p : Dm->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->W == Wm[m]) P = p;
+    if (!P) { P = new StdProfile(); Dm->prof.push_back(P); }
+    if (!P->uploaded || !same_profile(*P, fresh[m])) {
+      const StdProfile &F = fresh[m];
+      P->st = F.st; P->esc = F.esc; P->rtmpl = F.rtmpl; P->dtmpl = F.dtmpl;
+      P->R = F.R; P->RB = F.RB; P->root_dn = F.root_dn; P->root_dx = F.root_dx; P->root_cnum = F.root_cnum;
+      memcpy(P->root_child_base, F.root_child_base, sizeof(P->root_child_base));
+      memcpy(P->root_tsc, F.root_tsc, sizeof(P->root_tsc));
+      P->local_begin = F.local_begin; P->el = F.el;
+      P->qdbidx = qdbidx; P->semi = semi; P->W = Wm[m];
+      if ((status = upload(P->d_st, P->st.data(), P->st.size() * sizeof(DState))) != eslOK ||
+          (status = upload(P->d_esc, P->esc.data(), P->esc.size() * 4)) != eslOK ||
+          (status = upload(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4, 4)) != eslOK ||
+          (status = upload(P->d_dtmpl, P->dtmpl.data(), P->dtmpl.size() * 4, 4)) != eslOK)
+        return infx_fail(status, errbuf, "%s: device upload failed: %s", fname, hipGetErrorString(g_last_hip));
+      P->uploaded = true;
+    }
+    prof[m] = P;
+  }
+  /* launch-wide buffers live with the first model's scan matrix */
+  int m0 = 0;
+  while (m0 < nmodel && !prof[m0]) m0++;
+  SmxDev *D = (SmxDev *) smxs[m0]->dev;
+  if (semi == SEMI_FINSIDE && D->d_lsum_f.cap == 0) {
+    std::vector<float> t(INFX_LOGSUM_PAD, 0.f);
+    memcpy(t.data(), infx_flogsum_table(), LOGSUM_TBL * 4);
+    if ((status = upload(D->d_lsum_f, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+  }
+  if (semi == SEMI_IINSIDE && D->d_lsum_i.cap == 0) {
+    std::vector<int> t(INFX_LOGSUM_PAD, 0);
+    memcpy(t.data(), infx_ilogsum_table(), LOGSUM_TBL * 4);
+    if ((status = upload(D->d_lsum_i, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
+  }
+
+  /* launch shape: LDS accumulators for the widest root band */
+  int acc_max = 0, R_max = 0, RB_max = 0, M_max = 0, W_max = 0;
+  for (int m = 0; m < nmodel; m++) {
+    if (!prof[m]) continue;
+    const StdProfile *P = prof[m];
+    acc_max = ESL_MAX(acc_max, (P->root_dn <= P->root_dx) ? P->root_dx - P->root_dn + 1 : 0);
+    R_max = ESL_MAX(R_max, P->R); RB_max = ESL_MAX(RB_max, P->RB);
+    M_max = ESL_MAX(M_max, cms[m]->M); W_max = ESL_MAX(W_max, Wm[m]);
+  }
+  int wpb = 0, occ = 0;
+  for (int w = 16; w >= 1; w >>= 1) {
+    if (infx_wpb_override() && w != infx_wpb_override()) continue;
+    if (infx_std_lds_bytes(semi, 1, w, acc_max, 0) > (size_t) g_lds_max) continue;
+    const int o = infx_std_batch_occupancy(semi, w, acc_max);
+    if (o * w > occ * wpb) { wpb = w; occ = o; }
+  }
+  if (wpb < 1 || occ < 1)
+    return infx_fail(eslEINVAL, errbuf, "%s: no workgroup shape fits (W = %d, LDS limit %zu bytes)", fname, W_max, (size_t) g_lds_max);
+  const int64_t resident = (int64_t) occ * wpb * g_ncu;
+
+  /* chunks: each at least 2W of its model long; the costliest (rows x
+   * cells per row) first, so the long ones do not trail the launch */
+  struct Ch { int64_t a, b, lo; int m; double cost; };
+  std::vector<Ch> ch;
+  std::vector<double> cpr(nmodel, 0.0);   /* cells per residue at full band */
+  for (int m = 0; m < nmodel; m++) if (prof[m]) { double c = 0; infx_count_scan_cells(cms[m], smxs[m], qdbidx, 0, 1, Wm[m], &c); cpr[m] = c / Wm[m]; }
+  int64_t base_chunk = infx_chunk_override() > 0 ? infx_chunk_override() : (Ltot + resident - 1) / resident;
+  for (int w = 0; w < nwin; w++) {
+    const int m = wm[w];
+    const int64_t chunk = ESL_MAX(ESL_MAX(base_chunk, (int64_t) 2 * Wm[m]), (int64_t) 64);
+    for (int64_t a = wi[w]; a <= wj[w]; a += chunk) {
+      const int64_t b = ESL_MIN(wj[w], a + chunk - 1);
+      const int64_t s0 = ESL_MAX(wi[w], a - Wm[m] + 1);
+      ch.push_back(Ch{a, b, wi[w], m, (double) (b - s0 + 1) * cpr[m]});
+    }
+  }
+  std::stable_sort(ch.begin(), ch.end(), [](const Ch &x, const Ch &y) { return x.cost > y.cost; });
+  if (ch.size() > (size_t) INT32_MAX / 4) return infx_fail(eslERANGE, errbuf, "%s: too many chunks", fname);
+  const int nchunks = (int) ch.size();
+  std::vector<int64_t> ctab((size_t) nchunks * 4);
+  for (int c = 0; c < nchunks; c++) { ctab[4 * c] = ch[c].a; ctab[4 * c + 1] = ch[c].b; ctab[4 * c + 2] = ch[c].lo; ctab[4 * c + 3] = ch[c].m; }
+
+  /* one slot layout for every model: the largest deck, ring and vsc */
+  const size_t G = (size_t) ((W_max + 128 + 63) / 64) * 64;
+  const size_t deck_off = G * 4;
+  const size_t row_off  = deck_off + ((size_t) RB_max + G) * 4;
+  const size_t vsc_off  = row_off + ((size_t) 4 * R_max + G) * 4;
+  const size_t slot_bytes = ((vsc_off + (size_t) M_max * 4) + 255) / 256 * 256;
+  size_t mem_free = 0, mem_total = 0;
+  if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
+  const size_t scratch_budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
+                                + D->d_scratch.cap;
+  int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
+  nslots = (int) ESL_MIN((int64_t) nslots, resident);
+  if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+  const int64_t Lspan = span_hi - span_lo + 1;
+  int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 4 * Ltot, ((int64_t) 8 << 30) / (int64_t) sizeof(Cand)));
+  cand_cap = ESL_MAX(cand_cap, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
+  if ((status = D->d_dsq.reserve((size_t) Lspan + 128)) != eslOK ||
+      (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
+      (status = D->d_ncand.reserve(64)) != eslOK ||
+      (status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK ||
+      (status = D->d_chunk.reserve(((((size_t) nchunks * 4 + 255) / 256) * 256) + (size_t) nchunks * 16 + 64)) != eslOK)
+    return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
+  cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
+  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
+      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + span_lo, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
+
+  /* each model's parameters, then the launch's */
+  std::vector<ScanArgs> args(nmodel);
+  for (int m = 0; m < nmodel; m++) {
+    ScanArgs &A = args[m];
+    memset(&A, 0, sizeof(A));
+    const StdProfile *P = prof[m] ? prof[m] : prof[m0];
+    const CM_t *cm = prof[m] ? cms[m] : cms[m0];
+    A.st = (const DState *) P->d_st.ptr; A.nst = (int) P->st.size(); A.esc = P->d_esc.ptr;
+    A.dsq = (const uint8_t *) D->d_dsq.ptr + 64; A.dsq_off = span_lo;
+    A.row_tmpl = P->d_rtmpl.ptr; A.deck_tmpl = P->d_dtmpl.ptr;
+    A.lsum = (semi == SEMI_FINSIDE) ? D->d_lsum_f.ptr : (semi == SEMI_IINSIDE) ? D->d_lsum_i.ptr : NULL;
+    A.scratch = (char *) D->d_scratch.ptr; A.slot_bytes = (int64_t) slot_bytes;
+    A.deck_off = (int64_t) deck_off; A.row_off = (int64_t) row_off; A.vsc_off = (int64_t) vsc_off;
+    A.acc_n = acc_max;
+    A.M = cm->M; A.W = prof[m] ? Wm[m] : Wm[m0]; A.R = P->R; A.RB = P->RB;
+    A.root_dn = P->root_dn; A.root_dx = P->root_dx; A.root_cnum = P->root_cnum;
+    memcpy(A.root_child_base, P->root_child_base, sizeof(A.root_child_base));
+    memcpy(A.root_tsc, P->root_tsc, sizeof(A.root_tsc));
+    A.local_begin = P->local_begin; A.el_selfsc = P->el;
+    const int greedy = !(cm->search_opts & CM_SEARCH_CMNOTGREEDY);
+    A.cand_mode = (hitlists == NULL || hitlists[m] == NULL) ? 0 : (greedy ? 1 : 2);
+    A.cutoff = (hitlists == NULL || hitlists[m] == NULL) ? INFINITY : cutoffs[m];
+    A.env_cutoff = INFINITY;
+    A.cand = (Cand *) D->d_cand.ptr; A.ncand = (unsigned long long *) D->d_ncand.ptr; A.cand_cap = cand_cap;
+    A.chunk_best = (float *) D->d_chunk.ptr;
+    A.chunk_envi = (int64_t *) ((char *) D->d_chunk.ptr + (((size_t) nchunks * 4 + 255) / 256) * 256);
+    A.chunk_envj = A.chunk_envi + nchunks;
+  }
+  ScanArgs Am = args[m0];
+  Am.nchunks = nchunks; Am.nslots = nslots; Am.acc_n = acc_max;
+  if ((status = upload(D->d_ctab, ctab.data(), ctab.size() * sizeof(int64_t))) != eslOK)
+    return infx_fail(eslEMEM, errbuf, "%s: chunk table upload failed", fname);
+  Am.chunk_tab = (const int64_t *) D->d_ctab.ptr;
+
+  dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
+  unsigned long long ncand = 0;
+  float kernel_ms_total = 0;
+  int launches = 0;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    if ((status = upload(D->d_btab, args.data(), args.size() * sizeof(ScanArgs))) != eslOK)
+      return infx_fail(eslEMEM, errbuf, "%s: parameter upload failed", fname);
+    Am.args_tab = (const ScanArgs *) D->d_btab.ptr;
+    if (hipMemsetAsync(Am.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
+    hipEventRecord(g_ev0, g_stream);
+    if (upload(D->d_args, &Am, sizeof(Am)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
+    hipError_t e = infx_launch_std_batch(semi, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_max);
+    hipEventRecord(g_ev1, g_stream);
+    if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
+    if (hipMemcpyAsync(&ncand, Am.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess)
+      return infx_fail(eslESYS, errbuf, "%s: kernel execution failed: %s", fname, hipGetErrorString(hipGetLastError()));
+    float ms = 0; hipEventElapsedTime(&ms, g_ev0, g_ev1);
+    kernel_ms_total += ms; launches++;
+    if ((int64_t) ncand <= cand_cap) break;
+    if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer allocation failed", fname);
+    cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
+    for (auto &A : args) { A.cand = (Cand *) D->d_cand.ptr; A.cand_cap = cand_cap; }
+    Am.cand = (Cand *) D->d_cand.ptr; Am.cand_cap = cand_cap;
+  }
+  if ((int64_t) ncand > cand_cap) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer overflow", fname);
+
+  std::vector<Cand> hc(ncand);
+  std::vector<float> cbest(nchunks);
+  if ((ncand && hipMemcpyAsync(hc.data(), Am.cand, ncand * sizeof(Cand), hipMemcpyDeviceToHost, g_stream) != hipSuccess) ||
+      hipMemcpyAsync(cbest.data(), Am.chunk_best, nchunks * 4, hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+      hipStreamSynchronize(g_stream) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "%s: result download failed", fname);
+  if (ret_sc) {
+    for (int m = 0; m < nmodel; m++) ret_sc[m] = IMPOSSIBLE;
+    for (int c = 0; c < nchunks; c++) ret_sc[ch[c].m] = ESL_MAX(ret_sc[ch[c].m], cbest[c]);
+  }
+  if (hitlists) {
+    std::vector<std::vector<INFX_CAND>> per(nmodel);
+    for (size_t k = 0; k < ncand; k++) {
+      const int m = (int) ((uint32_t) hc[k].r >> 16);
+      if (m >= nmodel) return infx_fail(eslEINCONCEIVABLE, errbuf, "%s: candidate of model %d", fname, m);
+      INFX_CAND x; x.j = hc[k].j; x.d = hc[k].d; x.sc = hc[k].sc; x.r = hc[k].r & 0xffff; x.mode = TRMODE_J;
+      per[m].push_back(x);
+    }
+    for (int m = 0; m < nmodel; m++) {
+      if (!hitlists[m] || !prof[m]) continue;
+      const int greedy = !(cms[m]->search_opts & CM_SEARCH_CMNOTGREEDY);
+      if ((status = infx_resolve_hits(cms[m], errbuf, PLI_PASS_STD_ANY, 0, per[m].data(), (int64_t) per[m].size(), dsq, lo_m[m], hi_m[m],
+                                      Wm[m], cutoffs[m], 0, greedy, hitlists[m])) != eslOK)
+        return status;
+    }
+  }
+  double cells = 0;
+  for (int w = 0; w < nwin; w++) {
+    double cw = 0;
+    infx_count_scan_cells(cms[wm[w]], smxs[wm[w]], qdbidx, 0, wi[w], wj[w], &cw);
+    cells += cw;
+  }
+  double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (getenv("INFX_VERBOSE"))
+    fprintf(stderr, "[infx] %s batch: %d models, %d windows, %d chunks, %d slots x %zu B, wpb %d blocks/CU %d: wall %.1f ms, kernel %.1f ms, %llu candidates\n",
+            fname, nmodel, nwin, nchunks, nslots, slot_bytes, wpb, occ, wall, kernel_ms_total, (unsigned long long) ncand);
+  infx_timing_add(wall, kernel_ms_total, (int64_t) cells, launches);
+  return eslOK;
 }
 
 extern "C" int FastIInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,

@@ -78,7 +78,8 @@ struct ScanArgs {
   int            M, W, R, RB;
   int64_t        i0, j0;
   int64_t        chunk_len;
-  const int64_t *chunk_tab; /* null: chunk c covers i0 + c*chunk_len ..; else [3c..3c+2] = first row, last row, window start */
+  const int64_t *chunk_tab; /* null: chunk c covers i0 + c*chunk_len ..; else [3c..3c+2] = first row, last row, window start
+                               (batched launches: [4c..4c+3], the fourth the model's index in args_tab) */
   int            nchunks, nslots;
   int            acc_n;       /* local-begin accumulators per row and wavefront */
   /* root state */
@@ -100,6 +101,7 @@ struct ScanArgs {
   int64_t       *chunk_envi;  /* [nchunks] */
   int64_t       *chunk_envj;  /* [nchunks] */
   int            want_vsc;
+  const struct ScanArgs *args_tab;   /* batched launches: each model's parameters */
 };
 
 /* ---------------------------------------------------------------------- */

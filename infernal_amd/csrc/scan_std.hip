@@ -578,7 +578,7 @@ __device__ __forceinline__ int begin_argmax(const DState *stp, int nst, const ty
   return br;
 }
 
-template <int S, bool VSC, bool TBL_LDS, int NBC = 1>
+template <int S, bool VSC, bool TBL_LDS, int NBC = 1, bool BATCH = false>
 __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 {
   /* blocks per step of the generic states (gen_state2) */
@@ -621,21 +621,30 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
     /* chunk c reports rows ac..bc; its DP starts W-1 rows earlier, but not
      * before the start lo of its window (one window: the whole target) */
     int64_t ac, bc, lo;
-    if (const int64_t *tab = fresh(ap)->chunk_tab) {
+    /* the model's parameters: the launch's own, or (batched launches of
+     * several models) the chunk's model's; the slot layout, the chunk loop
+     * and the per-chunk outputs stay the launch's */
+    CArgs *cp = ap;
+    int bm = 0;
+    if (BATCH) {
+      const int64_t *tab = fresh(ap)->chunk_tab;
+      ac = tab[4 * c]; bc = tab[4 * c + 1]; lo = tab[4 * c + 2]; bm = (int) tab[4 * c + 3];
+      cp = (CArgs *) fresh(ap)->args_tab + bm;
+    } else if (const int64_t *tab = fresh(ap)->chunk_tab) {
       ac = tab[3 * c]; bc = tab[3 * c + 1]; lo = tab[3 * c + 2];
     } else {
       ac = fresh(ap)->i0 + (int64_t) c * fresh(ap)->chunk_len;
       bc = min(fresh(ap)->j0, ac + fresh(ap)->chunk_len - 1);
       lo = fresh(ap)->i0;
     }
-    const int64_t s0 = max(lo, ac - (int64_t) fresh(ap)->W + 1);
-    const int R = fresh(ap)->R;
+    const int64_t s0 = max(lo, ac - (int64_t) fresh(cp)->W + 1);
+    const int R = fresh(cp)->R;
 
     {
-      const T *rtmpl = (const T *) fresh(ap)->row_tmpl;
+      const T *rtmpl = (const T *) fresh(cp)->row_tmpl;
       for (int x = lane; x < R; x += 64) { T t = rtmpl[x]; rows[x] = t; rows[R + x] = t; rows[2 * R + x] = t; rows[3 * R + x] = t; }
-      const T *dtmpl = (const T *) fresh(ap)->deck_tmpl;
-      const int RB = fresh(ap)->RB;
+      const T *dtmpl = (const T *) fresh(cp)->deck_tmpl;
+      const int RB = fresh(cp)->RB;
       for (int x = lane; x < RB; x += 64) deck[x] = dtmpl[x];
     }
     wave_sync();
@@ -649,19 +658,19 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
       T *cur0 = rows + (size_t) ((j0 + 0) & 3) * R;
       T *cur1 = rows + (size_t) ((j0 + 1) & 3) * R;
       T *prv0 = rows + (size_t) ((j0 + 3) & 3) * R;   /* row j0-1 */
-      const uint8_t *dsq_j = fresh(ap)->dsq + (j0 + 1 - fresh(ap)->dsq_off);
+      const uint8_t *dsq_j = fresh(cp)->dsq + (j0 + 1 - fresh(cp)->dsq_off);
       const int xj0 = __builtin_amdgcn_readfirstlane((int) dsq_j[-1]);
       const int xj1 = two ? __builtin_amdgcn_readfirstlane((int) dsq_j[0]) : 0;
-      const int rdn = fresh(ap)->root_dn;
-      const int rdx0 = min(jp0, fresh(ap)->root_dx);
-      const int rdx1 = two ? min(jp1, fresh(ap)->root_dx) : rdn - 1;
+      const int rdn = fresh(cp)->root_dn;
+      const int rdx0 = min(jp0, fresh(cp)->root_dx);
+      const int rdx1 = two ? min(jp1, fresh(cp)->root_dx) : rdn - 1;
       const bool vs0 = j0 >= ac, vs1 = two && j0 + 1 >= ac;
-      const int local_begin = fresh(ap)->local_begin;
-      const uint32_t el = fresh(ap)->el_selfsc;
-      const DState *stp = fresh(ap)->st;
-      const T *esc = (const T *) fresh(ap)->esc;
-      const int nst = fresh(ap)->nst;
-      const int dsq_w = fresh(ap)->W + 64;
+      const int local_begin = fresh(cp)->local_begin;
+      const uint32_t el = fresh(cp)->el_selfsc;
+      const DState *stp = fresh(cp)->st;
+      const T *esc = (const T *) fresh(cp)->esc;
+      const int nst = fresh(cp)->nst;
+      const int dsq_w = fresh(cp)->W + 64;
       const uint8_t *dsq_b0 = dsq_j - dsq_w;   /* left residue of cell d of row j0: dsq_b0[dsq_w - d] */
 
       if (local_begin) {
@@ -773,7 +782,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         const T *cur = t ? cur1 : cur0;
         const T *acc = t ? acc1 : acc0;
         const int rdx = t ? rdx1 : rdx0;
-        CArgs *aq = fresh(ap);
+        CArgs *aq = fresh(cp);
         const int root_cnum = aq->root_cnum;
         const int cand_mode = aq->cand_mode;
         const float cutoff = aq->cutoff, env_cutoff = aq->env_cutoff;
@@ -828,7 +837,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
               const unsigned long long below = m & ((1ull << lane) - 1ull);
               const unsigned long long idx = basei + __popcll(below);
               if ((int64_t) idx < aq->cand_cap) {
-                Cand cd; cd.j = (int32_t) j; cd.d = d; cd.sc = bs; cd.r = br;
+                Cand cd; cd.j = (int32_t) j; cd.d = d; cd.sc = bs; cd.r = BATCH ? (br | (bm << 16)) : br;
                 aq->cand[idx] = cd;
               }
             }
@@ -863,6 +872,15 @@ __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_
 template <int S, bool VSC>
 __global__ void __launch_bounds__(64 * WPB_MAX) std_inside_lds_kernel(const ScanArgs *a)
 { std_scan_body<S, VSC, true>(a); }
+
+/* batched launches: chunks of several models' windows in one launch
+ * (chunk table rows of four: first row, last row, window start, model) */
+__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_cyk_batch_kernel(const ScanArgs *a)
+{ std_scan_body<SEMI_CYK, false, false, 1, true>(a); }
+
+template <int S>
+__global__ void __launch_bounds__(64 * WPB_MAX) std_inside_batch_kernel(const ScanArgs *a)
+{ std_scan_body<S, false, true, 1, true>(a); }
 
 template <int S, bool VSC>
 __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_inside_l2_kernel(const ScanArgs *a)
@@ -922,6 +940,39 @@ extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int g
   const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds, gacc, wide);
   if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
   const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n, gacc);
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  if (e != hipSuccess) return e;
+  void *args[] = { (void *) &a };
+  e = hipLaunchKernel(f, grid, block, args, lds, stream);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+/* batched launches of several models (CYK; Inside with the table in LDS) */
+static const void *std_batch_kernel_ptr(int semi)
+{
+  if (semi == SEMI_CYK) return (const void *) std_cyk_batch_kernel;
+  if (semi == SEMI_FINSIDE) return (const void *) std_inside_batch_kernel<SEMI_FINSIDE>;
+  if (semi == SEMI_IINSIDE) return (const void *) std_inside_batch_kernel<SEMI_IINSIDE>;
+  return NULL;
+}
+
+extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n)
+{
+  const void *f = std_batch_kernel_ptr(semi);
+  if (!f || wpb < 1 || wpb > WPB_MAX) return 0;
+  const size_t lds = infx_std_lds_bytes(semi, 1, wpb, acc_n, 0);
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
+  return nb;
+}
+
+extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a, int acc_n)
+{
+  const void *f = std_batch_kernel_ptr(semi);
+  if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
+  const size_t lds = infx_std_lds_bytes(semi, 1, block.x / 64, acc_n, 0);
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
   void *args[] = { (void *) &a };

@@ -175,6 +175,30 @@ def test_contract_errors():
     assert e.value.status == ia.eslEINVAL
 
 
+def test_batch_contract_errors():
+    """infx_FastScanBatch checks its windows and every model before any
+    device work: model indices in range, each model's windows sorted and
+    disjoint, the Inside flag, residue codes."""
+    cms = [ia.CM.read(os.path.join(CMDIR, n)).configure(local=True) for n in ("tRNA.c.cm", "Vault.c.cm")]
+    dsq = ia.dsq_from_codes(np.zeros(400, dtype=np.uint8))
+    cut = [0.0, 0.0]
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanBatch("cyk", cms, 1, dsq, [(1, 100, 0), (50, 150, 2)], cut)
+    assert e.value.status == ia.eslEINVAL
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanBatch("cyk", cms, 1, dsq, [(1, 100, 0), (50, 150, 0)], cut)
+    assert e.value.status == ia.eslEINCOMPAT and "sorted and disjoint" in str(e.value)
+    bad = dsq.copy()
+    bad[120] = 17
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanBatch("cyk", cms, 1, bad, [(1, 100, 0), (50, 150, 1)], cut)
+    assert e.value.status == ia.eslEINVAL
+    if not ia.gpu_available():
+        with pytest.raises(ia.ScanError) as e:      # windows of different models may overlap
+            ia.ScanBatch("finside", cms, 2, dsq, [(1, 100, 0), (50, 150, 1), (200, 390, 0)], cut)
+        assert e.value.status == ia.eslENODEVICE
+
+
 @pytest.mark.skipif(ia.gpu_available(), reason="only meaningful without a GPU")
 def test_no_gpu_fails_loudly():
     cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
