@@ -133,6 +133,61 @@ def cyk_1500(args, ia, synth, threads):
     return out
 
 
+def batch_leg(args, ia, synth):
+    """Many models in one launch: every model under tests/golden/cm (local,
+    FastCYKScan QDB1), --batch-windows windows of 1.2 W each per model,
+    sampled from a 2nd-order Markov target with each model's emitted
+    homologs planted.  infx_FastScanBatch (one launch for all models) against
+    one infx_FastScanWindows launch per model: total cells/s, and the
+    per-model latency distribution of the per-model launches (every model
+    of the batched launch finishes with the launch)."""
+    cmdir = os.path.join(ROOT, "tests", "golden", "cm")
+    names = sorted(f for f in os.listdir(cmdir) if f.endswith(".cm"))
+    cms = []
+    for nm in names:
+        cm = ia.CM.read(os.path.join(cmdir, nm))
+        cm.configure(local=True)
+        cms.append(cm)
+    rng = np.random.default_rng(404)
+    frags = []
+    for nm in names:
+        frags += synth.homologs(ia.CM.read(os.path.join(cmdir, nm)), 2, seed=int(rng.integers(1 << 30)))
+    n = int(args.batch_mb * 1e6)
+    bg = synth.markov2_background(n, gc=0.45, seed=405)
+    codes, _ = synth.plant(bg, frags, seed=406)
+    dsq = ia.dsq_from_codes(codes)
+    wins, per = [], []
+    for m, cm in enumerate(cms):
+        wl = int(1.2 * cm.W)
+        starts = np.sort(rng.choice(np.arange(1, n - wl, wl + 1), args.batch_windows, replace=False))
+        ws = [(int(a), int(a) + wl - 1) for a in starts]
+        per.append(ws)
+        wins += [(i, j, m) for (i, j) in ws]
+    cut = [0.0] * len(cms)
+    ia.ScanBatch("cyk", cms, 1, dsq, wins[:1], cut)                      # first-use set-up
+    t0 = time.perf_counter()
+    got = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
+    tb = time.perf_counter() - t0
+    cells = ia.last_call_timing()["cells"]
+    lat, same = [], True
+    for m, cm in enumerate(cms):
+        ia.ScanWindows("cyk", cm, 1, dsq, per[m][:1], 0.0)
+        t1 = time.perf_counter()
+        r = ia.ScanWindows("cyk", cm, 1, dsq, per[m], 0.0)
+        lat.append(time.perf_counter() - t1)
+        key = lambda h: (h[0], h[1])
+        same = same and sorted(r["hits"], key=key) == sorted(got[m]["hits"], key=key) and \
+            np.float32(r["sc"]).view(np.uint32) == np.float32(got[m]["sc"]).view(np.uint32)
+    log("batched: %d models, %d windows: one launch %.2f s, one launch per model %.2f s" % (len(cms), len(wins), tb, sum(lat)))
+    return {"models": ["%s (W %d)" % (nm[:-3], cm.W) for nm, cm in zip(names, cms)], "windows": len(wins),
+            "windows_per_model": args.batch_windows, "window_len": "1.2 W", "target_mb": args.batch_mb, "cells": cells,
+            "batched": {"wall_s": round(tb, 3), "gcells_per_s": round(cells / tb / 1e9, 3)},
+            "per_model_launches": {"wall_s": round(sum(lat), 3), "gcells_per_s": round(cells / sum(lat) / 1e9, 3),
+                                   "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+                                   "latency_p99_s": round(float(np.percentile(lat, 99)), 3)},
+            "batched_latency_s": round(tb, 3), "hits_identical": bool(same)}
+
+
 def trunc_leg(args, ia, synth, threads):
     """Truncated scanning with the 1,500-nt model: --trunc-windows windows of
     200-2,000 nt (uniform), half an emitted homolog clipped by the window at
@@ -317,6 +372,10 @@ def main():
                     help="longest truncated-scan window (a window is one wavefront's serial scan: time grows with its cells)")
     ap.add_argument("--trunc-batch", type=int, default=16, help="windows per truncated-scan launch")
     ap.add_argument("--trunc-only", action="store_true", help="run only the truncated-scan leg and print its JSON")
+    ap.add_argument("--batch-windows", type=int, default=0,
+                    help="windows per model of the many-models leg (one launch for all models vs one per model); 0 = skip")
+    ap.add_argument("--batch-mb", type=float, default=10.0, help="target the many-models leg samples its windows from")
+    ap.add_argument("--batch-only", action="store_true", help="run only the many-models leg and print its JSON")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -346,6 +405,9 @@ def main():
         ia.set_launch_params(args.chunk, 0)
     if args.trunc_only:
         print(json.dumps(trunc_leg(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))))
+        return
+    if args.batch_only:
+        print(json.dumps(batch_leg(args, ia, synth)))
         return
 
     cm = ia.CM.read(args.cm)
@@ -459,6 +521,8 @@ def main():
             trunc = {"error": "truncated leg exited with status %d" % pr.returncode}
             log("truncated leg failed: status %d" % pr.returncode)
 
+    batch = batch_leg(args, ia, synth) if args.batch_windows > 0 else None
+
     out = {
         "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
         "value": round(value, 3),
@@ -482,6 +546,7 @@ def main():
         "learned_gate": gate,
         "cyk_1500nt": big,
         "truncated_1500nt": trunc,
+        "many_models": batch,
     }
     print(json.dumps(out))
 
