@@ -912,6 +912,7 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     if (semi != SEMI_CYK && !smx->ints_valid)   return infx_fail(eslEINCOMPAT, errbuf, "%s, smx->ints_valid if FALSE", fname);
     if (cm->qdbinfo == NULL)            return infx_fail(eslEINCOMPAT, errbuf, "%s, cm->qdbinfo == NULL\n", fname);
     if (qdbidx < 0 || qdbidx > 2)       return infx_fail(eslEINCOMPAT, errbuf, "%s, qdbidx is invalid", fname);
+    if (cm->M > 65535)                  return infx_fail(eslERANGE, errbuf, "%s, model %d: %d states (a candidate keeps its state in 16 bits)", fname, m, cm->M);
     if (lmax_m[m] == 0) continue;       /* a model without windows */
     Wm[m] = (smx->W > lmax_m[m]) ? (int) lmax_m[m] : smx->W;
     if ((status = build_std_profile(cm, smx, qdbidx, semi, Wm[m], fresh[m], errbuf)) != eslOK) return status;
