@@ -179,13 +179,25 @@ def batch_leg(args, ia, synth):
         same = same and sorted(r["hits"], key=key) == sorted(got[m]["hits"], key=key) and \
             np.float32(r["sc"]).view(np.uint32) == np.float32(got[m]["sc"]).view(np.uint32)
     log("batched: %d models, %d windows: one launch %.2f s, one launch per model %.2f s" % (len(cms), len(wins), tb, sum(lat)))
+    tuned = None
+    if args.batch_tune:
+        from infernal_amd import autotune
+        best, trials = autotune.tune("cyk", cms, 1, dsq, wins, cut, sample=0.1)
+        with autotune.apply(best, max(cm.W for cm in cms)):
+            t0 = time.perf_counter()
+            got2 = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
+            tt = time.perf_counter() - t0
+        same = same and all(sorted(a["hits"]) == sorted(b["hits"]) for a, b in zip(got, got2))
+        tuned = {"config": best, "wall_s": round(tt, 3), "gcells_per_s": round(cells / tt / 1e9, 3),
+                 "trials": [[c, round(t, 4)] for c, t in trials]}
+        log("batched, tuned %s: %.2f s" % (best, tt))
     return {"models": ["%s (W %d)" % (nm[:-3], cm.W) for nm, cm in zip(names, cms)], "windows": len(wins),
             "windows_per_model": args.batch_windows, "window_len": "1.2 W", "target_mb": args.batch_mb, "cells": cells,
             "batched": {"wall_s": round(tb, 3), "gcells_per_s": round(cells / tb / 1e9, 3)},
             "per_model_launches": {"wall_s": round(sum(lat), 3), "gcells_per_s": round(cells / sum(lat) / 1e9, 3),
                                    "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
                                    "latency_p99_s": round(float(np.percentile(lat, 99)), 3)},
-            "batched_latency_s": round(tb, 3), "hits_identical": bool(same)}
+            "batched_latency_s": round(tb, 3), "tuned": tuned, "hits_identical": bool(same)}
 
 
 def trunc_leg(args, ia, synth, threads):
@@ -376,6 +388,7 @@ def main():
                     help="windows per model of the many-models leg (one launch for all models vs one per model); 0 = skip")
     ap.add_argument("--batch-mb", type=float, default=10.0, help="target the many-models leg samples its windows from")
     ap.add_argument("--batch-only", action="store_true", help="run only the many-models leg and print its JSON")
+    ap.add_argument("--batch-tune", action="store_true", help="many-models leg: also run with autotuned launch parameters")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
