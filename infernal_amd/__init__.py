@@ -340,6 +340,8 @@ def ScanBatch(algo, cms, qdbidx, dsq, windows, cutoffs, greedy=True):
     ScanWindows over that model's windows."""
     semi = {"cyk": 0, "finside": 1, "iinside": 2}[algo]
     n = len(cms)
+    if len(cutoffs) != n:
+        raise ValueError("ScanBatch: one cutoff per model (%d models, %d cutoffs)" % (n, len(cutoffs)))
     for cm in cms:
         c = cm.c
         if semi:
