@@ -195,14 +195,14 @@ int infx_wpb_override(void) { return g_wpb_override; }
 int DevBuf::reserve(size_t n)
 {
   if (n <= cap) return eslOK;
-  if (ptr) hipFree(ptr);
+  if (ptr) (void) hipFree(ptr);   /* contents are not kept; a failed free leaves nothing to undo */
   ptr = NULL; cap = 0;
   size_t want = n + n / 4 + 256;
   if (hipMalloc(&ptr, want) != hipSuccess) { ptr = NULL; return eslEMEM; }
   cap = want;
   return eslOK;
 }
-void DevBuf::release() { if (ptr) hipFree(ptr); ptr = NULL; cap = 0; }
+void DevBuf::release() { if (ptr) (void) hipFree(ptr); ptr = NULL; cap = 0; }
 
 struct StdProfile {
   int qdbidx, semi, W;
@@ -713,15 +713,16 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   int launches = 0;
   for (int attempt = 0; attempt < 3; attempt++) {
     if (hipMemsetAsync(A.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
-    hipEventRecord(g_ev0, g_stream);
+    (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
     if (upload(D->d_args, &A, sizeof(A)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
     hipError_t e = infx_launch_std(semi, A.want_vsc, tbl_lds, gacc, wide, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_n);
-    hipEventRecord(g_ev1, g_stream);
+    (void) hipEventRecord(g_ev1, g_stream);
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, A.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
         hipStreamSynchronize(g_stream) != hipSuccess)
       return infx_fail(eslESYS, errbuf, "%s: kernel execution failed: %s", fname, hipGetErrorString(hipGetLastError()));
-    float ms = 0; hipEventElapsedTime(&ms, g_ev0, g_ev1);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, g_ev0, g_ev1) != hipSuccess) { (void) hipGetLastError(); ms = 0; }
     kernel_ms_total += ms; launches++;
     if ((int64_t) ncand <= cand_cap) break;
     if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer allocation failed", fname);
@@ -1073,15 +1074,16 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
       return infx_fail(eslEMEM, errbuf, "%s: parameter upload failed", fname);
     Am.args_tab = (const ScanArgs *) D->d_btab.ptr;
     if (hipMemsetAsync(Am.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
-    hipEventRecord(g_ev0, g_stream);
+    (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
     if (upload(D->d_args, &Am, sizeof(Am)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
     hipError_t e = infx_launch_std_batch(semi, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_max);
-    hipEventRecord(g_ev1, g_stream);
+    (void) hipEventRecord(g_ev1, g_stream);
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, Am.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
         hipStreamSynchronize(g_stream) != hipSuccess)
       return infx_fail(eslESYS, errbuf, "%s: kernel execution failed: %s", fname, hipGetErrorString(hipGetLastError()));
-    float ms = 0; hipEventElapsedTime(&ms, g_ev0, g_ev1);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, g_ev0, g_ev1) != hipSuccess) { (void) hipGetLastError(); ms = 0; }
     kernel_ms_total += ms; launches++;
     if ((int64_t) ncand <= cand_cap) break;
     if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer allocation failed", fname);

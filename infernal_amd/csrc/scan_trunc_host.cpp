@@ -356,23 +356,29 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
   unsigned long long ncand = 0;
   float kms = 0; int launches = 0;
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0); hipEventCreate(&e1);
+  struct Events {   /* kernel timing; destroyed on every return */
+    hipEvent_t a = NULL, b = NULL;
+    ~Events() { if (a) (void) hipEventDestroy(a); if (b) (void) hipEventDestroy(b); }
+  } ev;
+  if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "%s: event creation failed", fname);
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   for (int attempt = 0; attempt < 3; attempt++) {
     if (hipMemsetAsync(A.ncand, 0, 128, stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
-    hipEventRecord(e0, stream);
+    (void) hipEventRecord(e0, stream);   /* timing only */
     hipError_t e = infx_launch_tr(semi, want_vsc, grid, block, stream, &A);
-    hipEventRecord(e1, stream);
+    (void) hipEventRecord(e1, stream);
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, A.ncand, 8, hipMemcpyDeviceToHost, stream) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess)
       return infx_fail(eslESYS, errbuf, "%s: kernel execution failed", fname);
-    float ms = 0; hipEventElapsedTime(&ms, e0, e1); kms += ms; launches++;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { (void) hipGetLastError(); ms = 0; }
+    kms += ms; launches++;
     if ((int64_t) ncand <= cand_cap) break;
     if (D->d_cand.reserve((size_t) ncand * sizeof(TrCand) + 4096) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer", fname);
     cand_cap = (int64_t) (D->d_cand.cap / sizeof(TrCand));
     A.cand = (TrCand *) D->d_cand.ptr; A.cand_cap = cand_cap;
   }
-  hipEventDestroy(e0); hipEventDestroy(e1);
   if ((int64_t) ncand > cand_cap) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer overflow", fname);
   cands.resize(ncand);
   std::vector<float> wb(nwin); std::vector<int> wm(nwin); std::vector<int64_t> wi(nwin), wj(nwin);
