@@ -49,6 +49,20 @@ def ops_per_res(cm, qdbidx, inside):
     return ops + 7.0 * red if inside else ops
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count (for the baseline's record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(cm, codes, threads, sample_len, W):
     """Oracle CYK (QDB1) + FInside (QDB2) on `threads` windows of sample_len+W-1."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -496,7 +510,7 @@ def main():
         log("CPU baseline: CYK %.1f s, FInside %.1f s in %d threads" % (r["cyk"][1], r["finside"][1], threads))
         cc = r["cyk"][0] + r["finside"][0]
         ct = r["cyk"][1] + r["finside"][1]
-        cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port", "host": host_cpu(),
                "sample": "oracle FastCYKScan (QDB1) + FastFInsideScan (QDB2), %d windows of %d nt in %d threads, "
                          "%.1f s CYK + %.1f s FInside" % (threads, args.cpu_sample + W - 1, threads, r["cyk"][1], r["finside"][1]),
                "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
