@@ -502,6 +502,10 @@ def main():
         if t.get("cm") == cm.c.name.decode():
             roof["traffic"] = t["hbm_bytes_per_launch"]
             roof["traffic_note"] = t.get("note", "")
+            # the measured bytes over this run's kernel time, against the HBM peak (8 TB/s):
+            # how far the dominant kernel is from being memory-bound
+            rate = t["hbm_bytes_per_launch"] / (kern[dom] / args.steps / 1e3) / 1e9
+            roof["traffic_rate"] = {"achieved": round(rate, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(rate / 8000.0, 4)}
 
     cpu = None
     if not args.no_cpu:
