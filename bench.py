@@ -49,6 +49,12 @@ def ops_per_res(cm, qdbidx, inside):
     return ops + 7.0 * red if inside else ops
 
 
+def traffic_rate(bytes_per_launch, kernel_ms):
+    """HBM bytes of one launch over its kernel time, against MI355X's 8 TB/s."""
+    rate = bytes_per_launch / (kernel_ms / 1e3) / 1e9
+    return {"achieved": round(rate, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(rate / 8000.0, 4)}
+
+
 def host_cpu():
     """The host's CPU model and logical CPU count (for the baseline's record)."""
     model = "unknown"
@@ -504,8 +510,7 @@ def main():
             roof["traffic_note"] = t.get("note", "")
             # the measured bytes over this run's kernel time, against the HBM peak (8 TB/s):
             # how far the dominant kernel is from being memory-bound
-            rate = t["hbm_bytes_per_launch"] / (kern[dom] / args.steps / 1e3) / 1e9
-            roof["traffic_rate"] = {"achieved": round(rate, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(rate / 8000.0, 4)}
+            roof["traffic_rate"] = traffic_rate(t["hbm_bytes_per_launch"], kern[dom] / args.steps)
 
     cpu = None
     if not args.no_cpu:
