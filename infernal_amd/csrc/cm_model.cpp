@@ -10,6 +10,7 @@
 #include <ctype.h>
 #include <string>
 #include <vector>
+#include <mutex>
 
 #include "cm_internal.h"
 
@@ -53,23 +54,24 @@ const int infx_ndegen[INFX_KP] = {1,1,1,1, 0, 2,2,2,2,2,2, 3,3,3,3, 4, 0,0};
 
 extern "C" int infx_abc_Digitize(const char *seq, int64_t L, ESL_DSQ *dsq)
 {
-  static int map[256];
-  static int built = 0;
-  if (!built) {
-    for (int c = 0; c < 256; c++) map[c] = -1;
+  /* built once, thread-safely (a function-local static) */
+  struct Map { int m[256]; };
+  static const Map map = [] {
+    Map t;
+    for (int c = 0; c < 256; c++) t.m[c] = -1;
     for (int x = 0; x < INFX_KP; x++) {
-      map[(unsigned char) infx_sym[x]] = x;
-      map[tolower((unsigned char) infx_sym[x])] = x;
+      t.m[(unsigned char) infx_sym[x]] = x;
+      t.m[tolower((unsigned char) infx_sym[x])] = x;
     }
-    map['T'] = map['t'] = 3;  /* T read as U */
-    map['X'] = map['x'] = 15; /* X read as N */
-    map['I'] = map['i'] = 0;  /* inosine read as A */
-    map['_'] = map['.'] = 4;  /* gap synonyms */
-    built = 1;
-  }
+    t.m['T'] = t.m['t'] = 3;  /* T read as U */
+    t.m['X'] = t.m['x'] = 15; /* X read as N */
+    t.m['I'] = t.m['i'] = 0;  /* inosine read as A */
+    t.m['_'] = t.m['.'] = 4;  /* gap synonyms */
+    return t;
+  }();
   dsq[0] = 255;
   for (int64_t i = 0; i < L; i++) {
-    int x = map[(unsigned char) seq[i]];
+    int x = map.m[(unsigned char) seq[i]];
     if (x < 0) return eslEINVAL;
     dsq[i + 1] = (ESL_DSQ) x;
   }
@@ -830,14 +832,14 @@ extern "C" int **ICalcInitDPScores(const CM_t *cm)
 
 static int   ilogsum_lookup[LOGSUM_TBL];
 static float flogsum_lookup[LOGSUM_TBL];
-static int   ilogsum_ready = 0, flogsum_ready = 0;
+static std::once_flag ilogsum_once, flogsum_once;   /* filled once, also when threads race */
 
 extern "C" void init_ilogsum(void)
 {
-  if (ilogsum_ready) return;
-  for (int i = 0; i < LOGSUM_TBL; i++)
-    ilogsum_lookup[i] = rint(INTSCALE * (sreLOG2(1. + sreEXP2((double) -i / INTSCALE))));
-  ilogsum_ready = 1;
+  std::call_once(ilogsum_once, [] {
+    for (int i = 0; i < LOGSUM_TBL; i++)
+      ilogsum_lookup[i] = rint(INTSCALE * (sreLOG2(1. + sreEXP2((double) -i / INTSCALE))));
+  });
 }
 extern "C" int ILogsum(int s1, int s2)
 {
@@ -847,10 +849,10 @@ extern "C" int ILogsum(int s1, int s2)
 }
 extern "C" void FLogsumInit(void)
 {
-  if (flogsum_ready) return;
-  for (int i = 0; i < LOGSUM_TBL; i++)
-    flogsum_lookup[i] = sreLOG2(1. + sreEXP2((double) -i / INTSCALE));
-  flogsum_ready = 1;
+  std::call_once(flogsum_once, [] {
+    for (int i = 0; i < LOGSUM_TBL; i++)
+      flogsum_lookup[i] = sreLOG2(1. + sreEXP2((double) -i / INTSCALE));
+  });
 }
 extern "C" float FLogsum(float s1, float s2)
 {
