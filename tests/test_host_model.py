@@ -247,3 +247,97 @@ def test_scan_profiles_build_for_every_model_and_length(name):
                     with pytest.raises(ia.ScanError) as e:
                         fn(cm, 1, p, dsq, 1, L, 0.0)
                     assert e.value.status == ia.eslENODEVICE, (name, local, L, fn.__name__, p, str(e.value))
+
+
+# IUPAC-IUB nucleotide codes (Cornish-Bowden 1985, Nucleic Acids Res 13:3021), U for T
+IUPAC = {"A": "A", "C": "C", "G": "G", "U": "U", "T": "U", "R": "AG", "Y": "CU", "M": "AC", "K": "GU",
+         "S": "CG", "W": "AU", "H": "ACU", "B": "CGU", "V": "ACG", "D": "AGU", "N": "ACGU", "X": "ACGU"}
+
+
+def test_degenerate_residue_scores_follow_iupac():
+    """Every IUPAC letter, upper or lower case, digitises to the code whose
+    optimised emission score is the mean of the canonical scores of the
+    residues the letter stands for: for singlet states, and for both sides
+    of a pair (esl_abc_FAvgScore, src/cm.c CalcOptimizedEmitScores).  The
+    gap code scores IMPOSSIBLE ('*' and '~', codes above 15, are refused by
+    the scanners).  A swapped row of the degeneracy table or a wrong symbol
+    order fails here."""
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    c = cm.c
+    Kp = 18
+    code = {ch: int(ia.digitize(ch)[1]) for ch in IUPAC}
+    for ch in IUPAC:
+        assert int(ia.digitize(ch.lower())[1]) == code[ch]
+    nuc = {ch: ["ACGU".index(y) for y in IUPAC[ch]] for ch in IUPAC}
+    codes_seen = set(code.values())
+    assert len(codes_seen) == 15          # X shares N's code, T shares U's
+    nsing = npair = 0
+    for v in range(c.M):
+        st = ord(c.sttype[v])
+        if st in (ia.ML_st, ia.MR_st, ia.IL_st, ia.IR_st):
+            nsing += 1
+            for ch, ys in nuc.items():
+                want = sum(c.esc[v][y] for y in ys) / len(ys)
+                assert abs(c.oesc[v][code[ch]] - want) < 1e-5, (v, ch)
+            assert c.oesc[v][4] < -9e35
+        elif st == ia.MP_st:
+            npair += 1
+            for a, la in nuc.items():
+                for b, rb in nuc.items():
+                    want = sum(c.esc[v][l * 4 + r] for l in la for r in rb) / (len(la) * len(rb))
+                    assert abs(c.oesc[v][code[a] * Kp + code[b]] - want) < 1e-4, (v, a, b)
+            assert all(c.oesc[v][4 * Kp + b] < -9e35 and c.oesc[v][b * Kp + 4] < -9e35 for b in range(16))
+    assert nsing > 10 and npair > 10
+
+
+def test_marginal_emission_scores(tmp_path):
+    """The truncated scanners' marginal scores (src/cm.c CMLogoddsify): for a
+    pair state the left marginal of residue x is the log-odds of the pair
+    probabilities summed over the right residue (and the right marginal over
+    the left one), recovered here from the pair scores; a degenerate code
+    scores the background-weighted mean over the residues its IUPAC letter
+    stands for (esl_abc_FExpectScVec).  Singlet states carry their own scores
+    on their side and 0 on the other.  The model's background is replaced by
+    a skewed one so that the weighting shows."""
+    with open(os.path.join(CMDIR, "Cobalamin.c.cm")) as f:
+        text = f.read()
+    text, n = re.subn(r"^NULL .*$", "NULL     0.500 -0.700  0.200 -0.300 ", text, count=1, flags=re.M)
+    assert n == 1
+    path = tmp_path / "Cobalamin.skewed.cm"
+    path.write_text(text)
+    cm = ia.CM.read(str(path)).configure(local=True, trunc=True)
+    c = cm.c
+    null = [c.null[x] for x in range(4)]
+    # the background as read: 0.25 * 2^x, renormalised (esl_vec_FNorm)
+    raw = [0.25 * 2.0 ** x for x in (0.5, -0.7, 0.2, -0.3)]
+    assert all(abs(null[x] - raw[x] / sum(raw)) < 1e-6 for x in range(4))
+    code = {ch: int(ia.digitize(ch)[1]) for ch in IUPAC}
+    nuc = {ch: ["ACGU".index(y) for y in IUPAC[ch]] for ch in IUPAC}
+
+    def expect(sc, ys):
+        return sum(null[y] * sc[y] for y in ys) / sum(null[y] for y in ys)
+
+    seen = set()
+    for v in range(c.M):
+        st = ord(c.sttype[v])
+        lm, rm = c.lmesc[v], c.rmesc[v]
+        if st == ia.MP_st:
+            for x in range(4):
+                pl = sum(2.0 ** c.esc[v][x * 4 + y] * null[x] * null[y] for y in range(4))
+                pr = sum(2.0 ** c.esc[v][y * 4 + x] * null[y] * null[x] for y in range(4))
+                assert abs(lm[x] - math.log2(pl / null[x])) < 1e-4, (v, x)
+                assert abs(rm[x] - math.log2(pr / null[x])) < 1e-4, (v, x)
+            sides = (lm, rm)
+        elif st in (ia.ML_st, ia.IL_st):
+            assert all(abs(lm[x] - c.esc[v][x]) < 1e-6 and rm[x] == 0.0 for x in range(4))
+            sides = (lm,)
+        elif st in (ia.MR_st, ia.IR_st):
+            assert all(abs(rm[x] - c.esc[v][x]) < 1e-6 and lm[x] == 0.0 for x in range(4))
+            sides = (rm,)
+        else:
+            continue
+        seen.add(st)
+        for sc in sides:
+            for ch, ys in nuc.items():
+                assert abs(sc[code[ch]] - expect(sc, ys)) < 1e-5, (v, ch)
+    assert {ia.MP_st, ia.ML_st, ia.MR_st, ia.IL_st, ia.IR_st} <= seen
