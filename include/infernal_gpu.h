@@ -392,6 +392,17 @@ void infx_last_call_timing(double *ret_wall_ms, double *ret_kernel_ms, int64_t *
  * arithmetic on the host. */
 int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int64_t L, void *out_st, int cap,
                            int *ret_R, int *ret_RB, int *ret_nesc, int *ret_root, char *errbuf);
+/* The device profile the truncated scanners would build (semi 0 RefTrCYKScan,
+ * 1 RefITrInsideScan) for pass pass_idx: up to st_cap 128-byte state
+ * descriptors (one per state), up to pen_cap 24-byte truncated-begin entries,
+ * the per-64-cell-block begin lists (offsets, then entries), each state's
+ * band of stored d cells (lohi[2v], lohi[2v+1]; lo > hi: none) and
+ * dims[0..6] = row cells, deck cells, T-row cells, emission entries, root
+ * band max, begin entries, begin-list length.  Returns the descriptor count,
+ * or -status.  No counterpart in the reference: used to check the truncated
+ * kernel's address arithmetic on the host. */
+int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, void *out_st, int st_cap,
+                          void *out_pen, int pen_cap, int *out_pblk, int pblk_cap, int *lohi, int *dims, char *errbuf);
 /* Query-dependent bands and W from the model's transition probabilities
  * (local ends off, standard local begins on).  Fills qdbinfo->dmin1/dmax1
  * (beta1) and dmin2/dmax2 (beta2) for every state and *ret_W (beta_W).

@@ -261,7 +261,10 @@ __global__ void __launch_bounds__(256) tr_scan_kernel(TrScanArgs a)
           if (!s->detached) {
             const int Ry0 = s->first_r;     /* 1 for IL (no self transition in R mode) */
             const bool chain = (kind == KIND_IL);
-            Cc carryJ = Jc[s->base + 0], carryL = Lc[s->base + 0];  /* d-1 = 0 cells: initial values */
+            /* d-1 = 0 cells: initial values of the IL chain (only IL bands
+             * hold a d = 0 cell; other left emitters never read one) */
+            Cc carryJ = R::imp(), carryL = R::imp();
+            if (chain) { carryJ = Jc[s->base + 0]; carryL = Lc[s->base + 0]; }
             for (int d0 = 1; d0 <= dx; d0 += 64) {
               const int d = d0 + lane;
               const bool act = d <= dx;

@@ -77,7 +77,8 @@ static void tr_d0_init(const CM_t *cm, int semi, std::vector<uint32_t> &a0)
   for (int v = 0; v < M; v++) a0[v] = (semi == TSEMI_IINSIDE) ? (uint32_t) n[v] : fb(f[v]);
 }
 
-static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pty, TrProfile &P, char *errbuf)
+static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pty, TrProfile &P, char *errbuf,
+                            std::vector<int> *ret_lo = nullptr, std::vector<int> *ret_hi = nullptr)
 {
   const int M = cm->M, W = trsmx->W;
   const int *dmax = qdbidx == SMX_QDB1_TIGHT ? cm->qdbinfo->dmax1 : qdbidx == SMX_QDB2_LOOSE ? cm->qdbinfo->dmax2 : NULL;
@@ -126,6 +127,8 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
       tr.push_back((d == 0 && cm->sttype[v] == E_st) ? zero : imp);
     }
   }
+  if (ret_lo) *ret_lo = lo;
+  if (ret_hi) *ret_hi = hi;
   while (rJ.size() % 64) { rJ.push_back(imp); rL.push_back(imp); rR.push_back(imp); }
   while (dJ.size() % 64) { dJ.push_back(imp); dL.push_back(imp); dR.push_back(imp); }
   P.R = (int) rJ.size(); P.RB = (int) dJ.size();
@@ -434,6 +437,33 @@ static int tr_scan_one(int semi, const char *fname, CM_t *cm, char *errbuf, CM_T
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   infx_timing_add(wall, 0, 0, 0);
   return eslOK;
+}
+
+/* The device profile RefTrCYKScan (semi 0) / RefITrInsideScan (semi 1)
+ * would upload for pass pass_idx, for checks of the kernel's address
+ * arithmetic on the host.  dims[0..6] = R, RB, RT, number of emission
+ * entries, root band max, number of truncated begins, length of the
+ * per-block begin lists; lohi[2v], lohi[2v+1] = the band of d cells state v
+ * holds in its rows (lo > hi: none).  Returns the number of descriptors, or
+ * -status on error. */
+extern "C" int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, void *out_st, int st_cap,
+                                     void *out_pen, int pen_cap, int *out_pblk, int pblk_cap, int *lohi, int *dims, char *errbuf)
+{
+  int pty;
+  if (!trsmx || qdbidx < 0 || qdbidx > 2 || (semi != TSEMI_CYK && semi != TSEMI_IINSIDE)) return -eslEINVAL;
+  if ((pty = cm_tr_penalties_IdxForPass(pass_idx)) == -1) return -eslEINCOMPAT;
+  if (cm->trp == NULL) return -eslEINCOMPAT;
+  TrProfile P;
+  std::vector<int> lo, hi;
+  int status = build_tr_profile(cm, trsmx, qdbidx, semi, pty, P, errbuf, &lo, &hi);
+  if (status != eslOK) return -status;
+  for (int i = 0; i < (int) P.st.size() && i < st_cap; i++) memcpy((char *) out_st + (size_t) i * sizeof(TState), &P.st[i], sizeof(TState));
+  for (int i = 0; i < (int) P.pen.size() && i < pen_cap; i++) memcpy((char *) out_pen + (size_t) i * sizeof(TrPen), &P.pen[i], sizeof(TrPen));
+  for (int i = 0; i < (int) P.pblk.size() && i < pblk_cap; i++) out_pblk[i] = P.pblk[i];
+  for (int v = 0; v < cm->M; v++) { lohi[2 * v] = lo[v]; lohi[2 * v + 1] = hi[v]; }
+  dims[0] = P.R; dims[1] = P.RB; dims[2] = P.RT; dims[3] = (int) P.esc.size(); dims[4] = P.root_dx;
+  dims[5] = (int) P.pen.size(); dims[6] = (int) P.pblk.size();
+  return (int) P.st.size();
 }
 
 extern "C" int RefTrCYKScan(CM_t *cm, char *errbuf, CM_TR_SCAN_MX *trsmx, int qdbidx, int pass_idx, ESL_DSQ *dsq,
