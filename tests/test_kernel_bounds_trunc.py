@@ -109,13 +109,14 @@ class Checker:
             self._err("emission index [%d, %d] outside [0, %d) %s" % (idx.min(), idx.max(), self.p["nesc"], ctx))
 
 
-def replay(cm, p):
-    """Every access tr_scan_kernel makes for the states and the root, at the
-    widest band (d = 1..dx)."""
+def replay(cm, p, states=True):
+    """Every access tr_scan_kernel makes for the states (when `states`) and
+    the root, at the widest band (d = 1..dx).  The states' part does not
+    depend on the truncation pass (only the begin list does)."""
     c = cm.c
     chk = Checker(p)
     codes = np.arange(16)
-    for v in range(c.M - 1, 0, -1):
+    for v in range(c.M - 1, 0, -1) if states else ():
         s = p["st"][v]
         kind, em, dx = int(s["kind"]), int(s["emitmode"]), int(s["dx"])
         if kind == KIND_SKIP or dx < 1:
@@ -225,9 +226,9 @@ def test_trunc_kernel_addresses_in_bounds(name):
     for local in (True, False):
         cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=local, trunc=True, scanmx=True, trscanmx=True)
         for semi, qdb in ((0, 1), (1, 2), (0, 0)):
-            for pidx in PASSES:
+            for k, pidx in enumerate(PASSES):
                 p = tr_profile(cm, qdb, semi, pidx)
-                chk = replay(cm, p)
+                chk = replay(cm, p, states=(k == 0))
                 assert not chk.errors, (name, local, semi, qdb, pidx, chk.errors)
 
 
