@@ -448,7 +448,7 @@ void infx_set_launch_params(int chunk_len, int waves_per_block);
  * bf16 on the matrix cores with fp32 accumulation, and its training
  * kernels (width 9, dilation per layer).  It picks the stretches of a target
  * that go to the exact Inside scan; the plain scan remains the fallback and
- * the judge of its misses.
+ * the measure of what the gate misses.
  * Device pointers, default stream; return 0 or a HIP error code.
  * Activations are [position][channel]; a batch is equal-length segments of
  * `seg` positions laid end to end.  Weights: [128][kp] with kp = 64 (first
