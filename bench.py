@@ -4,20 +4,30 @@
 Workload (the GPU FastCYKScan + FastFInsideScan configuration): a covariance
 model scanned over a seeded synthetic target of --mb megabases (second-order
 Markov background at 40% GC with --homologs emitted homolog fragments of
-0.5-1.0x model length planted at seeded positions).  One step = FastCYKScan
-with the tight bands (QDB1, beta 1e-7) over the whole target, then
-FastFInsideScan with the loose bands (QDB2, beta 1e-15) over the whole target
-("plain method": Inside on every position, no filter).
+0.5-1.0x model length planted at seeded positions).  The target is cut into
+database blocks of --block-mb megabases (each after the first starts W-1
+residues early, the overlap cmsearch gives its sequence windows).  One step =
+one block: FastCYKScan with the tight bands (QDB1, beta 1e-7), then
+FastFInsideScan with the loose bands (QDB2, beta 1e-15), both over the whole
+block ("plain method": Inside on every position, no filter); steps cycle
+through the blocks.
 
-value = DP cells of both scans / wall time of the step (Gcells/s), with the
-target already resident in device memory; cells are (v, j, d) cells inside
-the QDB bands, E states excluded, counted exactly as the reference scanners
-compute them.
+value = DP cells of both scans / wall time of the timed steps (Gcells/s),
+with the target resident in device memory (pinned once before the steps);
+cells are (v, j, d) cells inside the QDB bands, E states excluded, counted
+exactly as the reference scanners compute them.  Each wall time includes the
+host's hit replay.
+
+A complete JSON line is printed after every timed step (the last one is the
+result); no timed step is begun after --budget-s seconds from the start, and
+`steps` is the number actually timed.  The CPU baseline runs before the steps.
+Optional legs run after the steps and print one more line with their records.
 
 The JSON line also carries
   roofline:     the dominant kernel's algorithmic VALU ops (2*cnum+1 per
                 non-B cell, 2 per bifurcation split, FLogsum counted as 8)
-                per second against MI355X's non-packed FP32 VALU peak, and
+                per second of its HIP-event kernel time, against MI355X's
+                published FP32 peak and its non-packed FP32 VALU rate, and
   cpu_baseline: the C oracle (a restatement of the reference scanners,
                 gcc -O3, no FMA) timed on this host on a bounded sample.
 """
@@ -35,6 +45,8 @@ sys.path.insert(0, ROOT)
 
 BG_SEED = 20261203   # the benchmark target's background chain and sample
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, non-packed FP32
+FP32_PEAK_TFLOPS = 157.3                        # MI355X published FP32 vector peak (packed FMA)
+WORKLOAD_TAG = "multiloop350_block12p5mb"       # names the traffic summary of this workload in profiles/
 
 
 def log(*a):
@@ -377,12 +389,46 @@ def _cells(cm, qdbidx, i0, j0, truncated=False):
     return out.value
 
 
+def cpu_share():
+    """Host cores this process may use: its CPU affinity, bounded by a cgroup
+    CPU quota when one is set, and by 16 (the GPU host's per-GPU CPU share).
+    Returns (cores, record)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    cores = aff if quota is None else max(1, min(aff, int(quota)))
+    rec = dict(host_cpu(), affinity=aff, cgroup_quota_cpus=quota, per_gpu_share=16)
+    return max(1, min(cores, 16)), rec
+
+
+def blocks_of(n, block, W):
+    """The target cut into database blocks of `block` residues, each after
+    the first starting W-1 residues early (the overlap cmsearch gives its
+    sequence windows, src/cmsearch.c:810), so every hit lies inside a block."""
+    out, a = [], 1
+    while a <= n:
+        b = min(n, a + block - 1)
+        out.append((max(1, a - (W - 1)), b))
+        a = b + 1
+    return out
+
+
 def main():
+    t_proc = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mb", type=float, default=100.0, help="target length in megabases")
+    ap.add_argument("--block-mb", type=float, default=12.5, help="database block scanned by one step (megabases)")
+    ap.add_argument("--budget-s", type=float, default=430.0,
+                    help="wall-clock seconds (from start) after which no further timed step is begun")
     ap.add_argument("--homologs", type=int, default=2000)
     ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "synth_multiloop350.cm"))
     ap.add_argument("--no-cpu", action="store_true")
@@ -391,17 +437,19 @@ def main():
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--gate-steps", type=int, default=6000, help="learned-gate training steps (0 = skip the gate leg)")
+    ap.add_argument("--gate-budget-s", type=float, default=470.0,
+                    help="run the gate leg only if it can start before this many seconds")
     ap.add_argument("--gate-train-mb", type=float, default=8.0, help="learned-gate training target in megabases")
     ap.add_argument("--gate-smooth", type=int, default=128, help="residues the gate's logits are averaged over")
     ap.add_argument("--gate-margin", type=float, default=0.5, help="logit margin below the threshold quantile")
     ap.add_argument("--gate-quantile", type=float, default=0.005, help="held-out hit quantile the threshold sits at")
-    ap.add_argument("--cyk1500-mb", type=float, default=20.0,
+    ap.add_argument("--cyk1500-mb", type=float, default=0.0,
                     help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
     ap.add_argument("--trunc-windows", type=int, default=0,
                     help="windows of the truncated-scan leg (1,500-nt model, 200 to --trunc-lmax nt each; "
                          "run in a child process); 0 = skip")
     ap.add_argument("--trunc-lmax", type=int, default=1000,
-                    help="longest truncated-scan window (a window is one wavefront's serial scan: time grows with its cells)")
+                    help="longest truncated-scan window")
     ap.add_argument("--trunc-batch", type=int, default=16, help="windows per truncated-scan launch")
     ap.add_argument("--trunc-only", action="store_true", help="run only the truncated-scan leg and print its JSON")
     ap.add_argument("--batch-windows", type=int, default=0,
@@ -412,6 +460,11 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the single-GPU share of a sharded scan; "
+              "sharding across GPUs is described but not built); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
 
     if args.copy_probe > 0:
         import torch
@@ -424,11 +477,6 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes}))
         return
 
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (one wavefront per sequence chunk on a single MI355X); "
-              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-
     import infernal_amd as ia
     from infernal_amd import synth
     if not ia.gpu_available():
@@ -436,8 +484,10 @@ def main():
         sys.exit(1)
     if args.chunk:
         ia.set_launch_params(args.chunk, 0)
+    cores, cpu_rec = cpu_share()
+    threads = args.cpu_threads or cores
     if args.trunc_only:
-        print(json.dumps(trunc_leg(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))))
+        print(json.dumps(trunc_leg(args, ia, synth, None if args.no_cpu else threads)))
         return
     if args.batch_only:
         print(json.dumps(batch_leg(args, ia, synth)))
@@ -457,70 +507,22 @@ def main():
     c1, _ = cm.cells_per_res(1)
     c2, _ = cm.cells_per_res(2)
     log("model %s clen %d M %d W %d; cells/res QDB1 %.0f QDB2 %.0f" % (cm.c.name.decode(), clen, M, W, c1, c2))
+    blocks = blocks_of(n, int(args.block_mb * 1e6), W)
+    nb = len(blocks)
+    # the target stays in device memory for the whole run: steps start with
+    # their residues resident (one upload here)
+    ia.pin_target(dsq)
+    log("%d blocks of %.1f Mb (W-1 overlap), target pinned in device memory" % (nb, args.block_mb))
 
-    # warmup on a slice (first-use allocation, profile upload, code load)
-    wl = min(n, 2_000_000)
-    for _ in range(args.warmup):
-        if args.only in ("both", "cyk"):
-            ia.FastCYKScan(cm, 1, dsq, 1, wl, 0.0, want_hits=True)
-        if args.only in ("both", "finside"):
-            ia.FastFInsideScan(cm, 2, dsq, 1, wl, 0.0, want_hits=True)
-
-    step_times, cells_total, kern = [], 0.0, {"cyk": 0.0, "finside": 0.0}
-    cells_by = {"cyk": 0.0, "finside": 0.0}
-    hits = {}
-    for s in range(args.steps):
-        t0 = time.perf_counter()
-        if args.only in ("both", "cyk"):
-            r = ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
-            tm = ia.last_call_timing()
-            kern["cyk"] += tm["kernel_ms"]; cells_by["cyk"] += tm["cells"]
-            hits["cyk"] = len(r["hits"])
-            log("step %d CYK: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (s, tm["wall_ms"] / 1e3, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
-        if args.only in ("both", "finside"):
-            r = ia.FastFInsideScan(cm, 2, dsq, 1, n, 0.0, want_hits=True)
-            tm = ia.last_call_timing()
-            kern["finside"] += tm["kernel_ms"]; cells_by["finside"] += tm["cells"]
-            hits["finside"] = len(r["hits"])
-            plain_hits = [(h[0], h[1], h[4]) for h in r["hits"]]
-            log("step %d FInside: %.2f s wall, %.2f s kernel, %d hits, best %.2f" % (s, tm["wall_ms"] / 1e3, tm["kernel_ms"] / 1e3, len(r["hits"]), r["sc"]))
-        step_times.append(time.perf_counter() - t0)
-    total_t = sum(step_times)
-    cells_total = cells_by["cyk"] + cells_by["finside"]
-    value = cells_total / total_t / 1e9
-
-    # roofline of the dominant kernel
-    dom = max(kern, key=lambda k: kern[k])
-    q, inside = (1, False) if dom == "cyk" else (2, True)
-    ops = ops_per_res(cm, q, inside) * n * args.steps
-    achieved = ops / (kern[dom] / 1e3) / 1e12
-    kname = {"cyk": "std_cyk_kernel<false>", "finside": "std_inside_lds_kernel<1, false>"}
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None, "kernel": kname[dom],
-            "kernel_ms_per_launch": round(kern[dom] / args.steps, 1),
-            "gcells_per_s_kernel": {k: round(cells_by[k] / (kern[k] / 1e3) / 1e9, 2) for k in kern if kern[k] > 0}}
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE summary of this workload (see profiles/)
-    tj = os.path.join(ROOT, "profiles", "traffic_%s_%dmb.json" % (dom, int(round(n / 1e6))))
-    if os.path.exists(tj):
-        with open(tj) as f:
-            t = json.load(f)
-        if t.get("cm") == cm.c.name.decode():
-            roof["traffic"] = t["hbm_bytes_per_launch"]
-            roof["traffic_note"] = t.get("note", "")
-            # the measured bytes over this run's kernel time, against the HBM peak (8 TB/s):
-            # how far the dominant kernel is from being memory-bound
-            roof["traffic_rate"] = traffic_rate(t["hbm_bytes_per_launch"], kern[dom] / args.steps)
-
+    # CPU baseline first, so that every line printed below carries it
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         r = cpu_baseline(cm, codes, threads, args.cpu_sample, W)
         log("CPU baseline: CYK %.1f s, FInside %.1f s in %d threads" % (r["cyk"][1], r["finside"][1], threads))
         cc = r["cyk"][0] + r["finside"][0]
         ct = r["cyk"][1] + r["finside"][1]
-        cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port", "host": host_cpu(),
-               "sample": "oracle FastCYKScan (QDB1) + FastFInsideScan (QDB2), %d windows of %d nt in %d threads, "
+        cpu = {"value": round(cc / ct / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": "port", "host": cpu_rec,
+               "sample": "oracle FastCYKScan (QDB1) + FastFInsideScan (QDB2), %d windows of %d nt of the target in %d threads, "
                          "%.1f s CYK + %.1f s FInside" % (threads, args.cpu_sample + W - 1, threads, r["cyk"][1], r["finside"][1]),
                "cyk_gcells_per_s": round(r["cyk"][0] / r["cyk"][1] / 1e9, 4),
                "finside_gcells_per_s": round(r["finside"][0] / r["finside"][1] / 1e9, 4),
@@ -529,19 +531,125 @@ def main():
                             "finside_gcells_per_s": round(r["finside_1core"][0] / r["finside_1core"][1] / 1e9, 4),
                             "sample": "the first of those windows, alone"}}
 
-    gate = None
+    do_cyk, do_ins = args.only in ("both", "cyk"), args.only in ("both", "finside")
+    block_hits = {}
+
+    def step(k, timed):
+        a, b = blocks[k % nb]
+        out = {}
+        if do_cyk:
+            r = ia.FastCYKScan(cm, 1, dsq, a, b, 0.0, want_hits=True)
+            tm = ia.last_call_timing()
+            out["cyk"] = (tm["kernel_ms"], tm["cells"], len(r["hits"]))
+        if do_ins:
+            r = ia.FastFInsideScan(cm, 2, dsq, a, b, 0.0, want_hits=True)
+            tm = ia.last_call_timing()
+            out["finside"] = (tm["kernel_ms"], tm["cells"], len(r["hits"]))
+            block_hits[k % nb] = [(h[0], h[1], h[4]) for h in r["hits"]]
+        return out
+
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        step(k, False)
+        log("warmup step %d (block %d): %.2f s" % (k, k % nb, time.perf_counter() - t0))
+
+    kname = {"cyk": "std_cyk_kernel", "finside": "std_inside_lds_kernel<SEMI_FINSIDE>"}
+    ops_res = {"cyk": ops_per_res(cm, 1, False), "finside": ops_per_res(cm, 2, True)}
+    kern = {"cyk": [], "finside": []}
+    cells_by = {"cyk": 0.0, "finside": 0.0}
+    res_by = 0
+    hits = {}
+    step_times = []
+    line = None
+    for s in range(args.steps):
+        if step_times and time.perf_counter() - t_proc + max(step_times) > args.budget_s:
+            log("wall budget: stopping after %d of %d timed steps" % (len(step_times), args.steps))
+            break
+        k = args.warmup + s
+        a, b = blocks[k % nb]
+        t0 = time.perf_counter()
+        out = step(k, True)
+        dt = time.perf_counter() - t0
+        step_times.append(dt)
+        res_by += b - a + 1
+        for alg, (kms, cells, nh) in out.items():
+            kern[alg].append(kms)
+            cells_by[alg] += cells
+            hits[alg] = hits.get(alg, 0) + nh
+        log("step %d (block %d, %d-%d): %.2f s; " % (s, k % nb, a, b, dt) +
+            ", ".join("%s kernel %.2f s" % (alg, v[0] / 1e3) for alg, v in out.items()))
+        total_t = sum(step_times)
+        ns = len(step_times)
+        cells_total = cells_by["cyk"] + cells_by["finside"]
+        dom = max((x for x in kern if kern[x]), key=lambda x: sum(kern[x]))
+        kms_avg = sum(kern[dom]) / len(kern[dom])
+        ops = ops_res[dom] * res_by
+        achieved = ops / (sum(kern[dom]) / 1e3) / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "peak_nonpacked_valu": round(VALU_PEAK_TOPS, 2), "frac_nonpacked_valu": round(achieved / VALU_PEAK_TOPS, 4),
+                "kernel": kname[dom], "kernel_ms_per_launch": round(kms_avg, 1),
+                "ops_per_residue": round(ops_res[dom], 1),
+                "gcells_per_s_kernel": {x: round(cells_by[x] / (sum(kern[x]) / 1e3) / 1e9, 2) for x in kern if kern[x]}}
+        tj = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, WORKLOAD_TAG))
+        if os.path.exists(tj):
+            with open(tj) as f:
+                tr = json.load(f)
+            if tr.get("cm") == cm.c.name.decode() and tr.get("block_residues") == blocks[1][1] - blocks[1][0] + 1:
+                roof["traffic"] = tr["hbm_bytes_per_launch"]
+                roof["traffic_note"] = tr.get("note", "")
+                roof["traffic_rate"] = traffic_rate(tr["hbm_bytes_per_launch"], kms_avg)
+        line = {
+            "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
+            "value": round(cells_total / total_t / 1e9, 3),
+            "unit": "Gcells/s",
+            "n_gpus": 1,
+            "steps": ns,
+            "warmup": args.warmup,
+            "ms_per_step": round(total_t / ns * 1e3, 1),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "FastCYKScan QDB1 + FastFInsideScan QDB2, one %.1f-Mb database block per step, of a %.0f-Mb "
+                                   "target (2nd-order Markov, 40%% GC, %d planted homologs; %d blocks with W-1 overlap), one strand; "
+                                   "residues resident in device memory" % (args.block_mb, n / 1e6, len(frags), nb),
+                       "cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), clen, M, W),
+                       "cells_per_step": cells_total / ns, "hits": hits, "steps_requested": args.steps},
+            "roofline": roof,
+            "cpu_baseline": cpu,
+        }
+        print(json.dumps(line), flush=True)
+    if line is None:
+        print("bench.py: no step was timed", file=sys.stderr)
+        sys.exit(1)
+    ia.unpin_target()
+    mean_step = sum(step_times) / len(step_times)
+
+    extra = {}
     if args.gate_steps > 0 and args.only == "both":
-        gate = gate_leg(args, ia, synth, args.cm, codes, dsq, plain_hits, total_t / args.steps)
-
-    big = None
+        if len(block_hits) < nb:
+            extra["learned_gate"] = {"skipped": "the timed and warm-up steps did not cover every block"}
+        elif time.perf_counter() - t_proc > args.gate_budget_s:
+            extra["learned_gate"] = {"skipped": "wall budget"}
+        else:
+            seen, plain_hits = set(), []
+            for k in range(nb):
+                for h in block_hits[k]:
+                    if (h[0], h[1]) not in seen:
+                        seen.add((h[0], h[1]))
+                        plain_hits.append(h)
+            # the plain method over the whole target: every block once, at the
+            # measured mean step time
+            gate = gate_leg(args, ia, synth, args.cm, codes, dsq, plain_hits, mean_step * nb)
+            extra["learned_gate"] = gate
+            extra["learned_gate_speedup"] = gate["speedup"] if gate.get("recall", 0) >= 0.99 else None
     if args.cyk1500_mb > 0:
-        big = cyk_1500(args, ia, synth, None if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
-
-    trunc = None
+        extra["cyk_1500nt"] = cyk_1500(args, ia, synth, None if args.no_cpu else threads)
     if args.trunc_windows > 0:
         # in a child process of its own: the leg's device buffers and host
-        # memory are released when it ends, and the JSON line above does not
-        # depend on it
+        # memory are released when it ends
         import subprocess
         cmd = [sys.executable, "-u", os.path.abspath(__file__), "--trunc-only", "--trunc-windows", str(args.trunc_windows),
                "--trunc-lmax", str(args.trunc_lmax), "--trunc-batch", str(args.trunc_batch)]
@@ -551,40 +659,13 @@ def main():
             cmd += ["--cpu-threads", str(args.cpu_threads)]
         pr = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
         lines = [x for x in pr.stdout.splitlines() if x.startswith("{")]
-        if pr.returncode == 0 and lines:
-            trunc = json.loads(lines[-1])
-        else:
-            trunc = {"error": "truncated leg exited with status %d" % pr.returncode}
-            log("truncated leg failed: status %d" % pr.returncode)
-
-    batch = batch_leg(args, ia, synth) if args.batch_windows > 0 else None
-
-    out = {
-        "metric": "Gcells/s QDB CYK+Inside scan on 1 GPU; learned-gate speedup at \u226599% hits kept",
-        "value": round(value, 3),
-        "unit": "Gcells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(total_t / args.steps * 1e3, 1),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "FastCYKScan QDB1 + FastFInsideScan QDB2 over %.0f Mb (2nd-order Markov, 40%% GC, %d planted homologs), one strand"
-                               % (n / 1e6, len(frags)),
-                   "cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), clen, M, W),
-                   "cells_per_step": cells_total / args.steps, "hits": hits},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "learned_gate_speedup": (gate["speedup"] if gate and gate.get("recall", 0) >= 0.99 else None),
-        "learned_gate": gate,
-        "cyk_1500nt": big,
-        "truncated_1500nt": trunc,
-        "many_models": batch,
-    }
-    print(json.dumps(out))
+        extra["truncated_1500nt"] = json.loads(lines[-1]) if (pr.returncode == 0 and lines) else \
+            {"error": "truncated leg exited with status %d" % pr.returncode}
+    if args.batch_windows > 0:
+        extra["many_models"] = batch_leg(args, ia, synth)
+    if extra:
+        line.update(extra)
+        print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

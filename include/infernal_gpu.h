@@ -379,6 +379,16 @@ int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int trunca
  * Library-level controls and introspection (no counterpart in the reference).
  */
 int  infx_gpu_available(char *errbuf);                 /* 1 if a gfx9 HIP device can be used */
+/* Keep a device copy of a target's residues dsq[1..L] (checked once here).
+ * A later scan of the same buffer (the same dsq pointer, i0..j0 inside
+ * 1..L) reads that copy in place instead of uploading its range, so a
+ * caller that scans one sequence block several times (CYK filter, Inside,
+ * truncated passes) moves it over PCIe once.  The caller must not change
+ * dsq[1..L] while it is pinned; pinning another buffer replaces it.
+ * eslEINVAL for a non-residue code, eslEMEM, eslENODEVICE.  No counterpart in
+ * the reference (its scanners read host memory). */
+int infx_target_Pin(const ESL_DSQ *dsq, int64_t L, char *errbuf);
+int infx_target_Unpin(void);
 /* Timing of the last scanner call on this thread: total wall ms around the
  * call, and the summed device duration of the DP kernel launches in it
  * (HIP events on the library's stream). */
@@ -403,6 +413,12 @@ int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int6
  * kernel's address arithmetic on the host. */
 int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, void *out_st, int st_cap,
                           void *out_pen, int pen_cap, int *out_pblk, int pblk_cap, int *lohi, int *dims, char *errbuf);
+/* Attach truncated-scanner device resources to trsmx as a GPU scan would
+ * (nprof host-side profiles, no device memory), without a device; FreeCM or
+ * cm_tr_scan_mx_Destroy must release them.  eslEINVAL if trsmx already has
+ * resources or nprof is outside 0..8.  No counterpart in the reference: lets
+ * a CPU test (under AddressSanitizer) check the truncated matrix's teardown. */
+int infx_debug_attach_trdev(CM_t *cm, CM_TR_SCAN_MX *trsmx, int nprof);
 /* Query-dependent bands and W from the model's transition probabilities
  * (local ends off, standard local begins on).  Fills qdbinfo->dmin1/dmax1
  * (beta1) and dmin2/dmax2 (beta2) for every state and *ret_W (beta_W).

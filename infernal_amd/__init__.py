@@ -142,6 +142,23 @@ def set_launch_params(chunk_len=0, waves_per_block=0):
     _lib.infx_set_launch_params(int(chunk_len), int(waves_per_block))
 
 
+def pin_target(dsq):
+    """infx_target_Pin: keep a device copy of dsq[1..L] (dsq as from
+    dsq_from_codes, sentinels at both ends); scans of this same array read it
+    in place.  Keep the array alive and unchanged while pinned."""
+    dsq = np.asarray(dsq)
+    if dsq.dtype != np.uint8 or not dsq.flags.c_contiguous:
+        raise ValueError("pin_target: dsq must be a contiguous uint8 array")
+    eb = C.create_string_buffer(ERRBUF)
+    st = _lib.infx_target_Pin(C.c_void_p(dsq.ctypes.data), C.c_int64(len(dsq) - 2), eb)
+    if st != eslOK:
+        raise ScanError(st, eb.value.decode(errors="replace"))
+
+
+def unpin_target():
+    _lib.infx_target_Unpin()
+
+
 def last_call_timing():
     w, k, c, n = C.c_double(), C.c_double(), C.c_int64(), C.c_int()
     _lib.infx_last_call_timing(C.byref(w), C.byref(k), C.byref(c), C.byref(n))

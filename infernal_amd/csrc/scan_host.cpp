@@ -114,7 +114,7 @@ extern "C" void cm_tr_scan_mx_Destroy(CM_t *cm, CM_TR_SCAN_MX *t)
 {
   (void) cm;
   if (!t) return;
-  infx_dev_release(t->dev);
+  infx_trdev_release(t->dev);   /* a truncated matrix owns a TrDev, never an SmxDev */
   free_bands(t->dnAAA); free_bands(t->dxAAA);
   free(t->bestsc); free(t->bestr); free(t->bestmode);
   free(t);
@@ -191,6 +191,56 @@ extern "C" void infx_set_launch_params(int chunk_len, int waves_per_block)
 int infx_chunk_override(void) { return g_chunk_override; }
 int infx_wpb_override(void) { return g_wpb_override; }
 
+/* ------------------------------------------------------------------------ */
+/* pinned target: one device copy of a caller's residues dsq[1..L], laid out
+ * as 64 zero bytes | residues | 64 zero bytes.  Scans of that same buffer
+ * (same dsq pointer, range inside 1..L) read it in place instead of
+ * uploading their range; the residues were checked once, when pinned.     */
+static const ESL_DSQ *g_pin_host = NULL;
+static int64_t        g_pin_L = 0;
+static void          *g_pin_dev = NULL;
+static size_t         g_pin_cap = 0;
+
+extern "C" int infx_target_Pin(const ESL_DSQ *dsq, int64_t L, char *errbuf)
+{
+  if (!dsq || L < 1) return infx_fail(eslEINVAL, errbuf, "infx_target_Pin: no residues");
+  for (int64_t x = 1; x <= L; x++)
+    if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "infx_target_Pin: residue code %d at %lld is not a residue", (int) dsq[x], (long long) x);
+  int status;
+  if ((status = infx_dev_init(errbuf)) != eslOK) return status;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const size_t need = (size_t) L + 128;
+  if (need > g_pin_cap) {
+    if (g_pin_dev) (void) hipFree(g_pin_dev);
+    g_pin_dev = NULL; g_pin_cap = 0; g_pin_host = NULL; g_pin_L = 0;
+    if (hipMalloc(&g_pin_dev, need) != hipSuccess) { g_pin_dev = NULL; return infx_fail(eslEMEM, errbuf, "infx_target_Pin: %zu bytes of device memory", need); }
+    g_pin_cap = need;
+  }
+  if (hipMemsetAsync(g_pin_dev, 0, need, g_stream) != hipSuccess ||
+      hipMemcpyAsync((char *) g_pin_dev + 64, dsq + 1, (size_t) L, hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+      hipStreamSynchronize(g_stream) != hipSuccess)
+    return infx_fail(eslESYS, errbuf, "infx_target_Pin: upload failed");
+  g_pin_host = dsq; g_pin_L = L;
+  return eslOK;
+}
+
+extern "C" int infx_target_Unpin(void)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_pin_dev) (void) hipFree(g_pin_dev);
+  g_pin_dev = NULL; g_pin_cap = 0; g_pin_host = NULL; g_pin_L = 0;
+  return eslOK;
+}
+
+/* Device address of residue x0 when dsq is the pinned target and x0..x1 lies
+ * inside it (reads reach 64 bytes either side: the pads), else NULL. */
+const uint8_t *infx_pinned(const ESL_DSQ *dsq, int64_t x0, int64_t x1)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_pin_dev || dsq != g_pin_host || x0 < 1 || x1 > g_pin_L || x0 > x1) return NULL;
+  return (const uint8_t *) g_pin_dev + 64 + (x0 - 1);
+}
+
 /* growable device buffer */
 int DevBuf::reserve(size_t n)
 {
@@ -221,12 +271,10 @@ struct StdProfile {
 struct SmxDev {
   std::vector<StdProfile *> prof;
   DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args, d_ctab, d_btab;
-  void  *trdev = NULL;   /* truncated-scanner resources */
   ~SmxDev() {
     for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
     d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release(); d_ctab.release();
     d_btab.release(); d_lsum_f.release(); d_lsum_i.release();
-    infx_trdev_release(trdev);
   }
 };
 
@@ -508,8 +556,9 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   if (i0 < 1)                         return infx_fail(eslEINCOMPAT, errbuf, "%s, i0 must be >= 1", fname);
   if (j0 - i0 + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
   int64_t L = 0, Lmax = 0;      /* residues scanned; the longest window */
+  const uint8_t *pinned = infx_pinned(dsq, i0, j0);   /* residues already on the device (checked when pinned) */
   for (int w = 0; w < nwin; w++) {
-    for (int64_t x = wi[w]; x <= wj[w]; x++)
+    for (int64_t x = wi[w]; !pinned && x <= wj[w]; x++)
       if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
     L += wj[w] - wi[w] + 1;
     Lmax = ESL_MAX(Lmax, wj[w] - wi[w] + 1);
@@ -639,7 +688,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
 
-  if ((status = D->d_dsq.reserve((size_t) Lspan + 128)) != eslOK ||
+  if ((status = D->d_dsq.reserve(pinned ? 64 : (size_t) Lspan + 128)) != eslOK ||
       (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
       (status = D->d_chunk.reserve((size_t) nchunks * 24 + 64)) != eslOK ||
       (status = D->d_ncand.reserve(64)) != eslOK)
@@ -653,8 +702,9 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   /* target residues i0..j0 behind and before 64 zero bytes (residue codes
    * past j0 are read, and ignored, by idle lanes of the last row pair) */
-  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
-      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+  if (!pinned &&
+      (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
+       hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess))
     return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
 
   ScanArgs A;
@@ -662,7 +712,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   A.st = (const DState *) P->d_st.ptr;
   A.nst = (int) P->st.size();
   A.esc = P->d_esc.ptr;
-  A.dsq = (const uint8_t *) D->d_dsq.ptr + 64;
+  A.dsq = pinned ? pinned : (const uint8_t *) D->d_dsq.ptr + 64;
   A.dsq_off = i0;
   A.row_tmpl = P->d_rtmpl.ptr;
   A.deck_tmpl = P->d_dtmpl.ptr;
@@ -899,7 +949,8 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     Ltot += wj[w] - wi[w] + 1;
   }
   if (span_hi - span_lo + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
-  for (int64_t x = span_lo; x <= span_hi; x++)
+  const uint8_t *pinned = infx_pinned(dsq, span_lo, span_hi);   /* residues already on the device (checked when pinned) */
+  for (int64_t x = span_lo; !pinned && x <= span_hi; x++)
     if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
   std::vector<int> Wm(nmodel, 0);
   std::vector<StdProfile> fresh(nmodel);
@@ -1020,15 +1071,16 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
   const int64_t Lspan = span_hi - span_lo + 1;
   int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 4 * Ltot, ((int64_t) 8 << 30) / (int64_t) sizeof(Cand)));
   cand_cap = ESL_MAX(cand_cap, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
-  if ((status = D->d_dsq.reserve((size_t) Lspan + 128)) != eslOK ||
+  if ((status = D->d_dsq.reserve(pinned ? 64 : (size_t) Lspan + 128)) != eslOK ||
       (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
       (status = D->d_ncand.reserve(64)) != eslOK ||
       (status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK ||
       (status = D->d_chunk.reserve(((((size_t) nchunks * 4 + 255) / 256) * 256) + (size_t) nchunks * 16 + 64)) != eslOK)
     return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
-  if (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
-      hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + span_lo, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+  if (!pinned &&
+      (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
+       hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + span_lo, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess))
     return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
 
   /* each model's parameters, then the launch's */
@@ -1039,7 +1091,7 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     const StdProfile *P = prof[m] ? prof[m] : prof[m0];
     const CM_t *cm = prof[m] ? cms[m] : cms[m0];
     A.st = (const DState *) P->d_st.ptr; A.nst = (int) P->st.size(); A.esc = P->d_esc.ptr;
-    A.dsq = (const uint8_t *) D->d_dsq.ptr + 64; A.dsq_off = span_lo;
+    A.dsq = pinned ? pinned : (const uint8_t *) D->d_dsq.ptr + 64; A.dsq_off = span_lo;
     A.row_tmpl = P->d_rtmpl.ptr; A.deck_tmpl = P->d_dtmpl.ptr;
     A.lsum = (semi == SEMI_FINSIDE) ? D->d_lsum_f.ptr : (semi == SEMI_IINSIDE) ? D->d_lsum_i.ptr : NULL;
     A.scratch = (char *) D->d_scratch.ptr; A.slot_bytes = (int64_t) slot_bytes;

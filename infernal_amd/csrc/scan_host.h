@@ -20,6 +20,7 @@ extern "C" void infx_trdev_release(void *trdev);
 extern "C" int  infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells);
 
 hipStream_t infx_stream(void);
+const uint8_t *infx_pinned(const uint8_t *dsq, int64_t x0, int64_t x1);
 int  infx_num_cu(void);
 int  infx_chunk_override(void);
 int  infx_wpb_override(void);

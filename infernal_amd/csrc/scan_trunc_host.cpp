@@ -466,6 +466,25 @@ extern "C" int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx,
   return (int) P.st.size();
 }
 
+/* Attach truncated-scanner resources to trsmx without touching a device: a
+ * TrDev holding nprof host-side profiles (descriptors and tables sized as for
+ * the model, device buffers empty), as a scan on the GPU leaves behind.
+ * Lets a CPU test free a CM whose truncated matrix carries them. */
+extern "C" int infx_debug_attach_trdev(CM_t *cm, CM_TR_SCAN_MX *trsmx, int nprof)
+{
+  if (!cm || !trsmx || trsmx->dev || nprof < 0 || nprof > 8) return eslEINVAL;
+  TrDev *D = new TrDev();
+  for (int k = 0; k < nprof; k++) {
+    TrProfile *P = new TrProfile();
+    P->qdbidx = k % 3; P->semi = k % 2;
+    P->st.resize(cm->M); P->esc.resize(16 * cm->M); P->rtmpl.resize(cm->M); P->dtmpl.resize(cm->M);
+    P->pen.resize(cm->M); P->pblk.resize(cm->M + 1);
+    D->prof.push_back(P);
+  }
+  trsmx->dev = D;
+  return eslOK;
+}
+
 extern "C" int RefTrCYKScan(CM_t *cm, char *errbuf, CM_TR_SCAN_MX *trsmx, int qdbidx, int pass_idx, ESL_DSQ *dsq,
                             int64_t i0, int64_t j0, float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
                             int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, char *ret_mode, float *ret_sc)

@@ -27,7 +27,7 @@ import math
 
 import numpy as np
 
-from . import CM, FastCYKScan, FastIInsideScan, dsq_from_codes
+from . import CM, FastCYKScan, ScanWindows, dsq_from_codes
 from .calibrate import read_exp_infos
 
 
@@ -51,18 +51,25 @@ def cyk_filter_windows(cm, dsq, n, exp_lc, F6=1e-4, null3=True):
 
 
 def final_stage(cm, dsq, windows, exp_li, T, null3=True):
-    """Inside hits (start, stop, score, P-value) in each window."""
-    out = []
-    for (i, j) in windows:
-        r = FastIInsideScan(cm, 2, dsq, i, j, T, do_null3=null3, want_hits=True)
-        for h in r["hits"]:
-            out.append((h[0], h[1], float(h[4]), float(exp_li.pvalue(h[4]))))
-    return out
+    """Inside hits (start, stop, score, P-value) in the windows: one
+    FastIInsideScan launch for all of them (infx_FastScanWindows; each
+    window's hits are those of one FastIInsideScan call on it)."""
+    if not windows:
+        return []
+    r = ScanWindows("iinside", cm, 2, dsq, windows, T, do_null3=null3)
+    hits = sorted(r["hits"], key=lambda h: (h[0], h[1]))
+    return [(h[0], h[1], float(h[4]), float(exp_li.pvalue(h[4]))) for h in hits]
+
+
+# complement of each residue code of the RNA alphabet "ACGU-RYMKSWHBVDN*~":
+# A<->U, C<->G, R<->Y, M<->K, H<->D, B<->V; gap, S, W, N, '*' and '~' map to
+# themselves (esl_sq_ReverseComplement with the alphabet's complement table)
+_COMPLEMENT = np.array([3, 2, 1, 0, 4, 6, 5, 8, 7, 9, 10, 14, 13, 12, 11, 15, 16, 17], dtype=np.uint8)
 
 
 def revcomp(codes):
-    """Reverse complement of residue codes 0..3 (A C G U)."""
-    return (3 - np.asarray(codes, dtype=np.uint8)[::-1]).astype(np.uint8)
+    """Reverse complement of residue codes (canonical and IUPAC degenerate)."""
+    return _COMPLEMENT[np.asarray(codes, dtype=np.uint8)[::-1]]
 
 
 def search(cm_path, codes, Z=None, E=10.0, F6=1e-4, both_strands=True, null3=True, cm=None):

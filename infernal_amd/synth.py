@@ -12,6 +12,8 @@ homologs sampled from a covariance model.
 * plant(background, homologs, seed): overwrite seeded, non-overlapping
   positions of the background with the homolog fragments.
 """
+import bisect
+
 import numpy as np
 
 from . import B_st, D_st, E_st, IL_st, IR_st, ML_st, MP_st, MR_st, S_st
@@ -55,25 +57,57 @@ def markov2_background(n, gc=0.40, seed=1, seg=10000, sample_seed=None):
     return out.reshape(-1)[:n].copy()
 
 
-def emit(cm, rng, max_len=100000):
-    """Sample one sequence (residue codes 0..3) from the model (global mode)."""
+def _emit_tables(cm):
+    """Per-state sampling tables, read from the model once and cached on the
+    Python CM object: state types, children, and for every state the
+    cumulative distributions of its transitions and of its emissions."""
+    tab = getattr(cm, "_emit_tab", None)
+    if tab is not None:
+        return tab
     c = cm.c
     M = c.M
     st = [ord(c.sttype[v]) for v in range(M)]
     cfirst = [c.cfirst[v] for v in range(M)]
     cnum = [c.cnum[v] for v in range(M)]
-    t = [[c.t[v][k] for k in range(6)] for v in range(M)]
-    e = [[c.e[v][k] for k in range(16)] for v in range(M)]
-    root_t = [c.root_trans[k] for k in range(cnum[0])] if c.root_trans else t[0][:cnum[0]]
 
-    def pick(p):
+    def cdf(p):
+        # the cumulative sums numpy's Generator.choice(len(p), p=p/sum(p))
+        # draws against, so that sampling by bisection below picks the same
+        # index for the same uniform draw
         p = np.asarray(p, dtype=np.float64)
-        s = p.sum()
-        return int(rng.choice(len(p), p=p / s))
+        p = p / p.sum()
+        q = p.cumsum()
+        q /= q[-1]
+        return q.tolist()
+
+    tcdf, ecdf = [None] * M, [None] * M
+    for v in range(M):
+        if st[v] in (E_st, B_st):
+            continue
+        probs = ([c.root_trans[k] for k in range(cnum[0])] if c.root_trans else [c.t[0][k] for k in range(cnum[0])]) \
+            if v == 0 else [c.t[v][k] for k in range(cnum[v])]
+        tcdf[v] = cdf(probs)
+        if st[v] in (ML_st, IL_st, MR_st, IR_st):
+            ecdf[v] = cdf([c.e[v][k] for k in range(4)])
+        elif st[v] == MP_st:
+            ecdf[v] = cdf([c.e[v][k] for k in range(16)])
+    tab = (st, cfirst, cnum, tcdf, ecdf)
+    cm._emit_tab = tab
+    return tab
+
+
+def emit(cm, rng, max_len=100000):
+    """Sample one sequence (residue codes 0..3) from the model (global mode).
+    Each choice takes one uniform draw from rng, as Generator.choice with
+    probabilities does."""
+    st, cfirst, cnum, tcdf, ecdf = _emit_tables(cm)
+    rand = rng.random
+
+    def pick(q):
+        return bisect.bisect_right(q, rand())
 
     # iterative walk; a node emits left residues on the way down and right
     # residues on the way back up
-    left, right_stack = [], []
     out = []
     stack = [("v", 0)]
     while stack:
@@ -89,15 +123,14 @@ def emit(cm, rng, max_len=100000):
             stack.append(("v", cfirst[v]))
             continue
         if s in (ML_st, IL_st):
-            out.append(pick(e[v][:4]))
+            out.append(pick(ecdf[v]))
         elif s in (MR_st, IR_st):
-            stack.append(("r", pick(e[v][:4])))
+            stack.append(("r", pick(ecdf[v])))
         elif s == MP_st:
-            xy = pick(e[v][:16])
+            xy = pick(ecdf[v])
             out.append(xy // 4)
             stack.append(("r", xy % 4))
-        probs = root_t if v == 0 else t[v][:cnum[v]]
-        k = pick(probs)
+        k = pick(tcdf[v])
         stack.append(("v", cfirst[v] + k))
         if len(out) > max_len:
             break

@@ -11,6 +11,10 @@ CMDIR = os.path.join(ROOT, "tests", "golden", "cm")
 _cache = {}
 
 
+def cm_path(name):
+    return os.path.join(CMDIR, name)
+
+
 def load_cm(name, local=True, trunc=False):
     key = (name, local, trunc)
     if key not in _cache:

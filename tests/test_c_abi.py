@@ -1,0 +1,54 @@
+"""The C ABI from a C caller: tests/c/abi_check.c is compiled with gcc
+against include/infernal_gpu.h alone and linked to libinfernal_gpu.so, as
+Infernal's cm_pipeline.c would be.
+
+* Without a GPU every scanner returns eslENODEVICE through its declared
+  prototype (argument checking and profile building still run).
+* The truncated scan matrix's teardown, under AddressSanitizer: the host code
+  is rebuilt with g++ -fsanitize=address (host objects only; the gfx950
+  kernel objects are the library's), truncated-scanner resources are attached
+  to a configured model as a GPU scan leaves them, and FreeCM must release
+  them without a heap error.  Before the fix, cm_tr_scan_mx_Destroy deleted
+  those resources as the standard matrix's type, and this run reported
+  new-delete-type-mismatch.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CDIR = os.path.join(ROOT, "tests", "c")
+CMDIR = os.path.join(ROOT, "tests", "golden", "cm")
+
+
+def build(target):
+    path = os.path.join(CDIR, target)
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", CDIR, target], check=True, capture_output=True)
+    return path
+
+
+def test_c_caller_builds_and_calls_every_scanner(tmp_path):
+    exe = build("abi_check")
+    seq = tmp_path / "seq.txt"
+    rng = np.random.default_rng(3)
+    seq.write_text("".join("ACGU"[x] for x in rng.integers(0, 4, 400)) + "\n")
+    r = subprocess.run([exe, "scan", os.path.join(CMDIR, "tRNA.c.cm"), str(seq)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    names = [ln.split()[0] for ln in r.stdout.splitlines() if not ln.startswith(" ")]
+    assert names == ["FastCYKScan", "FastFInsideScan", "FastIInsideScan", "RefTrCYKScan", "RefITrInsideScan"]
+    status = {int(ln.split()[2]) for ln in r.stdout.splitlines() if not ln.startswith(" ")}
+    assert status in ({0}, {100}), r.stdout
+
+
+def test_truncated_matrix_teardown_under_asan():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "infernal_amd", "csrc")], check=True, capture_output=True)
+    exe = os.path.join(CDIR, "abi_check_asan")
+    subprocess.run(["make", "-C", CDIR, "abi_check_asan"], check=True, capture_output=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1")
+    for name in ("tRNA.c.cm", "synth_nested1500.cm"):
+        r = subprocess.run([exe, "teardown", os.path.join(CMDIR, name)], capture_output=True, text=True, env=env, timeout=300)
+        assert r.returncode == 0 and "teardown ok" in r.stdout, r.stdout + r.stderr[-3000:]
+        assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]

@@ -17,6 +17,7 @@ import os
 import numpy as np
 import pytest
 
+import infernal_amd as ia
 from infernal_amd import calibrate as cal
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -135,6 +136,27 @@ def test_cyk_filter_window_merge(monkeypatch):
 def test_revcomp():
     from infernal_amd import pipeline as pl
     assert pl.revcomp(np.array([0, 1, 2, 3, 3], dtype=np.uint8)).tolist() == [0, 0, 1, 2, 3]
+
+
+def test_revcomp_degenerate_codes():
+    """Every IUPAC letter's complement stands for the Watson-Crick
+    complements of the residues the letter stands for (sets from the
+    IUPAC-IUB table), the complement is an involution, and a target with N
+    and other degenerate codes reverse-complements (the scanners reject codes
+    above 15, so a wrong table would make the minus strand fail)."""
+    from infernal_amd import pipeline as pl
+    from test_host_model import IUPAC
+    code = {ch: int(ia.digitize(ch)[1]) for ch in IUPAC}
+    wc = {"A": "U", "C": "G", "G": "C", "U": "A"}
+    by_set = {frozenset(IUPAC[ch]): code[ch] for ch in IUPAC}
+    for ch in IUPAC:
+        want = by_set[frozenset(wc[x] for x in IUPAC[ch])]
+        assert pl.revcomp(np.array([code[ch]], dtype=np.uint8))[0] == want, ch
+    allc = np.arange(18, dtype=np.uint8)
+    assert np.array_equal(pl.revcomp(pl.revcomp(allc)), allc)
+    seq = ia.digitize("ACGUNNRYKMSWBDHV")[1:-1]
+    rc = pl.revcomp(seq)
+    assert rc.max() <= 15 and len(rc) == len(seq)
 
 
 def test_exp_lines_round_trip(tmp_path):

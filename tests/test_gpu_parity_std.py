@@ -209,3 +209,27 @@ def test_scan_windows_equals_one_scan_per_window(algo):
     key = lambda h: (h[0], h[1])
     assert hits_equal(sorted(got["hits"], key=key), sorted(want, key=key))
     assert np.float32(got["sc"]).view(np.uint32) == np.float32(best).view(np.uint32)
+
+
+def test_pinned_target():
+    """A pinned target (infx_target_Pin): block scans read the device copy in
+    place, with real residues beyond each block's ends instead of zero pads,
+    and give the oracle's results for the block; the target's last block
+    reads into the pin's trailing pad.  Windowed scans of the pinned buffer
+    agree too.  Unpinned, the same calls upload their ranges again."""
+    cm = load_cm("tRNA.c.cm")
+    L = 9000
+    dsq = random_dsq(L, seed=41)
+    ia.pin_target(dsq)
+    try:
+        for (a, b) in ((1, 2500), (2301, 6000), (5801, L)):
+            _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, a, b, 0.0, env=3.0, vsc=True)
+            _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, 2, dsq, a, b, 0.0, env=3.0)
+        wins = [(11, 900), (1500, 1800), (8000, L)]
+        g = ia.ScanWindows("cyk", cm, 1, dsq, wins, 0.0)
+    finally:
+        ia.unpin_target()
+    u = ia.ScanWindows("cyk", cm, 1, dsq, wins, 0.0)
+    assert hits_equal(sorted(g["hits"]), sorted(u["hits"]))
+    assert np.float32(g["sc"]).view(np.uint32) == np.float32(u["sc"]).view(np.uint32)
+    _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 2301, 6000, 0.0, env=3.0)

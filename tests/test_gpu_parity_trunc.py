@@ -11,7 +11,7 @@ import pytest
 
 import infernal_amd as ia
 import oracle_bind as ob
-from helpers import load_cm, random_dsq, hits_equal, describe_hit_diff
+from helpers import load_cm, cm_path, random_dsq, hits_equal, describe_hit_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -86,3 +86,32 @@ def test_trunc_window_batch():
         assert hits_equal(got, o["hits"]), (w, describe_hit_diff(got, o["hits"]))
         assert (ei[w], ej[w]) == (o["envi"], o["envj"])
         lib.cm_tophits_Destroy(ths[w])
+
+
+def test_trunc_nested1500_clipped_windows():
+    """The truncated-scan workload's shape: the 1,500-nt nested-stem model
+    (W 1522) over windows of 200-320 nt, two holding an emitted homolog
+    clipped by the window, two random decoys.  RefTrCYKScan in the 5'-only
+    pass (QDB1) and RefITrInsideScan in the forced 5'-and-3' pass (QDB2), all
+    windows in one launch each, against the oracle per window (run in threads:
+    the oracle's truncated Inside takes ~10 s per window)."""
+    import concurrent.futures as cf
+    from infernal_amd import synth
+    cm = load_cm("synth_nested1500.cm", local=True, trunc=True)
+    codes, wins, hom = synth.truncated_windows(ia.CM.read(cm_path("synth_nested1500.cm")), 4, seed=301,
+                                               lmin=200, lmax=320, pool=2)
+    assert list(hom) == [True, False, True, False]
+    dsq = ia.dsq_from_codes(codes)
+    for algo, oalgo, q, p in (("cyk", ob.ORACLE_TRCYK, 1, ia.PLI_PASS_5P_ONLY_FORCE),
+                              ("iinside", ob.ORACLE_TRIINSIDE, 2, ia.PLI_PASS_5P_AND_3P_FORCE)):
+        print("case nested1500", algo, q, p, wins, flush=True)
+        g = ia.TrScanWindows(algo, cm, q, p, dsq, wins, -5.0, env_cutoff=2.0)
+        with cf.ThreadPoolExecutor(len(wins)) as ex:
+            os_ = list(ex.map(lambda w: ob.scan(cm, oalgo, q, dsq, w[0], w[1], -5.0, env_cutoff=2.0, pass_idx=p), wins))
+        for w, o in enumerate(os_):
+            assert np.float32(g["sc"][w]).view(np.uint32) == np.float32(o["sc"]).view(np.uint32), (algo, w, g["sc"][w], o["sc"])
+            assert g["mode"][w] == o["mode"], (algo, w)
+            assert hits_equal(g["hits"][w], o["hits"]), (algo, w, describe_hit_diff(g["hits"][w], o["hits"]))
+            assert (g["envi"][w], g["envj"][w]) == (o["envi"], o["envj"]), (algo, w)
+        # the clipped homologs score far above the decoys
+        assert min(g["sc"][0], g["sc"][2]) > max(g["sc"][1], g["sc"][3]) + 50.0, g["sc"]
