@@ -351,9 +351,21 @@ int cm_tr_penalties_IdxForPass(int pass_idx);
 int cm_pli_PassAllowsTruncation(int pass_idx);
 int cm_hit_AllowTruncation(CM_t *cm, int pass_idx, int64_t start, int64_t stop, int64_t i0, int64_t j0, char mode, int b);
 
-/* src/cm_dpsearch.c:3099 cm_CountSearchDPCalcs: DP calcs per residue (the
- * reference's definition), for forecasts.  Also: this library's cell count,
- * the number of (v,d) cells inside the band at steady state (jp >= W). */
+/* Replaces cm_CountSearchDPCalcs (src/cm_dpsearch.c:3099-3205): millions of
+ * DP calculations per residue of a standard scan of L residues under the
+ * bands dmin/dmax (both NULL: no bands), by the reference's definition (per j:
+ * 1 + (cnum+1) cells (+cells if emitting) for a non-B state, 1 + k range per
+ * B cell, (cnum+1) cells plus the local begins' cells for the root), float
+ * sums in the reference's order; per sub-model in *ret_vcalcs (caller frees),
+ * whole model in *ret_calcs.  cmcalibrate's forecast calls it
+ * (src/cmcalibrate.c:2434-2442). */
+int cm_CountSearchDPCalcs(CM_t *cm, char *errbuf, int L, int *dmin, int *dmax, int W, int correct_for_first_W,
+                          float **ret_vcalcs, float *ret_calcs);
+/* This library's own work measure (no counterpart in the reference; it is
+ * not cm_CountSearchDPCalcs): (v,d) cells inside the band per residue at
+ * steady state (jp >= W), E states excluded, and algorithmic VALU ops per
+ * residue (2*cnum + 1 if emitting per non-B cell, 2*(kmax-kmin+1) per B cell),
+ * the numerator of the benchmark's roofline. */
 int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells_per_res, double *ret_ops_per_res);
 /* As infx_CountSearchCells, plus the max (CYK) / log-sum (Inside) reductions
  * per residue: cnum per non-B cell, kmax-kmin+1 per B cell.  No counterpart

@@ -969,6 +969,75 @@ extern "C" int infx_CountSearchWork(CM_t *cm, int qdbidx, int truncated, double 
   return eslOK;
 }
 
+/* cm_CountSearchDPCalcs (src/cm_dpsearch.c:3099-3205): millions of DP
+ * calculations per residue that a standard scan of L residues does, per
+ * sub-model (ret_vcalcs[v], the caller frees it) and for the whole model
+ * (ret_calcs = vcalcs[0]).  The reference's definition of a calculation:
+ *   non-B state v >= 1, per j:  1 + (cnum+1) (dx-dn+1), plus (dx-dn+1) if v emits
+ *   B state, per d with kmax >= kmin:  1 + (kmax-kmin+1)
+ *   root, per j:  (cnum+1) (dx-dn+1), plus each finite local begin's dx-dn+1
+ * with dn = max(dmin, 1 or 2 for MP) and dx = min(j, dmax, W) (no bands:
+ * dmin/dmax NULL).  correct_for_first_W counts only j = W+1..L.  The sums are
+ * float, each term added as (float) n / 1e6 in the reference's j, v, d order,
+ * so the result rounds as the reference's does.  Each state then adds the
+ * sub-model below it (B: both children; others: state v+1), and every entry
+ * is divided by the residues counted. */
+extern "C" int cm_CountSearchDPCalcs(CM_t *cm, char *errbuf, int L, int *dmin, int *dmax, int W, int correct_for_first_W,
+                                     float **ret_vcalcs, float *ret_calcs)
+{
+  if (W > L && correct_for_first_W)
+    return infx_fail(eslFAIL, errbuf, "gross misuse of cm_CountSearchDPCalcs(), W: %d > L: %d and correct_for_first_W is TRUE.\n", W, L);
+  const bool banded = (dmin != NULL && dmax != NULL);
+  if (W > L) W = L;
+  const int M = cm->M;
+  float *vc = (float *) calloc((size_t) M, sizeof(float));
+  if (!vc) return infx_fail(eslEMEM, errbuf, "cm_CountSearchDPCalcs(): memory error.");
+  auto add = [](float &acc, long n) { acc += ((float) n) / 1000000.; };   /* float += double, rounded to float */
+  /* the d band of state v at end position j (cells d = dn..dx) */
+  auto band = [&](int v, int j, bool mp_floor, int *dn, int *dx) {
+    const int lo = (mp_floor && cm->sttype[v] == MP_st) ? 2 : 1;
+    *dn = banded ? ESL_MAX(dmin[v], lo) : lo;
+    *dx = ESL_MIN(j, W);
+    if (banded) *dx = ESL_MIN(*dx, dmax[v]);
+  };
+  const int Leff = correct_for_first_W ? (L - W) : L;
+  for (int j = correct_for_first_W ? W + 1 : 1; j <= L; j++) {
+    int dn, dx;
+    for (int v = M - 1; v > 0; v--) {
+      const int st = cm->sttype[v];
+      if (st == E_st) continue;
+      band(v, j, true, &dn, &dx);
+      if (st == B_st) {
+        const int w = cm->cfirst[v], y = cm->cnum[v];
+        for (int d = dn; d <= dx; d++) {
+          const int kmin = banded ? ESL_MAX(ESL_MAX(dmin[y], d - dmax[w]), 0) : 0;
+          const int kmax = banded ? ESL_MIN(dmax[y], d - dmin[w]) : d;
+          if (kmax >= kmin) add(vc[v], 1 + (kmax - kmin + 1));
+        }
+      } else if (dx >= dn) {
+        add(vc[v], 1 + (long) (cm->cnum[v] + 1) * (dx - dn + 1));
+        if (StateDelta(st) > 0) add(vc[v], dx - dn + 1);
+      }
+    }
+    band(0, j, false, &dn, &dx);
+    if (dx >= dn) add(vc[0], (long) (cm->cnum[0] + 1) * (dx - dn + 1));
+    if (cm->flags & CMH_LOCAL_BEGIN)
+      for (int y = 1; y < M; y++) {
+        /* no bands: dn = 1 for every begin state, MP included */
+        band(y, j, banded, &dn, &dx);
+        if (dx >= dn && NOT_IMPOSSIBLE(cm->beginsc[y])) add(vc[0], dx - dn + 1);
+      }
+  }
+  for (int v = M - 1; v >= 0; v--) {
+    if (cm->sttype[v] == B_st)      vc[v] += vc[cm->cnum[v]] + vc[cm->cfirst[v]];
+    else if (cm->sttype[v] != E_st) vc[v] += vc[v + 1];
+  }
+  for (int v = M - 1; v >= 0; v--) vc[v] /= Leff;
+  if (ret_calcs) *ret_calcs = vc[0];
+  if (ret_vcalcs) *ret_vcalcs = vc; else free(vc);
+  return eslOK;
+}
+
 extern "C" int infx_CountSearchCells(CM_t *cm, int qdbidx, int truncated, double *ret_cells, double *ret_ops)
 {
   return infx_CountSearchWork(cm, qdbidx, truncated, ret_cells, ret_ops, NULL);
