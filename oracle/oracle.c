@@ -497,3 +497,25 @@ void oracle_act_update(double *act, int W, int jp, ESL_DSQ x)
   for (int a = 0; a < INFX_K; a++) dst[a] = src[a];
   oracle_DCount(dst, x, 1.);
 }
+
+/* ------------------------------------------------------------------------ */
+/* direct access to single rules, for the tests that pin them                 */
+
+/* cm_hit_AllowTruncation as the hit reporters apply it */
+int oracle_allow_truncation(const CM_t *cm, int pass_idx, int64_t start, int64_t stop, int64_t i0, int64_t j0, int mode, int b)
+{
+  return oracle_allow_trunc(cm, pass_idx, start, stop, i0, j0, mode, b);
+}
+
+/* the NULL3 correction a reporter subtracts from a hit i..j of dsq: its
+ * composition from the running residue counts, as during a scan */
+float oracle_null3_hit(const CM_t *cm, const ESL_DSQ *dsq, int64_t i, int64_t j)
+{
+  oracle_tables_init();
+  const int W = (int) (j - i + 1);
+  double *act = (double *) calloc((size_t) (W + 1) * INFX_K, sizeof(double));
+  for (int jp = 1; jp <= W; jp++) oracle_act_update(act, W, jp, dsq[i + jp - 1]);
+  float c = null3_corr(cm, act, W, 1, W, W);
+  free(act);
+  return c;
+}

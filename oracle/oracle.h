@@ -57,6 +57,18 @@ int  oracle_scan(const CM_t *cm, int algo, int qdbidx, int pass_idx, const ESL_D
                  ORACLE_RESULT *res, char *errbuf);
 void oracle_result_free(ORACLE_RESULT *res);
 
+/* Hit accumulation driven directly (the reporters the scanners call per j):
+ * begin, one update per end position j with that row's bestsc/bestr[/bestmode]
+ * over d = dmin..dmax, finish (kept hits into res; res NULL: discard). */
+int  oracle_trunc_hits_begin(void **acc, int L, int64_t i0, int64_t j0, int W, int greedy);
+int  oracle_trunc_hits_update(void *acc, const CM_t *cm, int pass_idx, int j, int dmin, int dmax, const float *bestsc,
+                              const int *bestr, const char *bestmode, const double *act, float cutoff, int do_null3);
+int  oracle_trunc_hits_finish(void *acc, ORACLE_RESULT *res, char *errbuf);
+/* Single rules, for the tests that pin them: cm_hit_AllowTruncation, and the
+ * NULL3 correction of hit i..j computed as a reporter computes it. */
+int   oracle_allow_truncation(const CM_t *cm, int pass_idx, int64_t start, int64_t stop, int64_t i0, int64_t j0, int mode, int b);
+float oracle_null3_hit(const CM_t *cm, const ESL_DSQ *dsq, int64_t i, int64_t j);
+
 /* The oracle's own log-sum tables (src/logsum.c:74-148), for table checks. */
 const float *oracle_flogsum_table(void);
 const int   *oracle_ilogsum_table(void);

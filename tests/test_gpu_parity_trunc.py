@@ -113,5 +113,6 @@ def test_trunc_nested1500_clipped_windows():
             assert g["mode"][w] == o["mode"], (algo, w)
             assert hits_equal(g["hits"][w], o["hits"]), (algo, w, describe_hit_diff(g["hits"][w], o["hits"]))
             assert (g["envi"][w], g["envj"][w]) == (o["envi"], o["envj"]), (algo, w)
-        # the clipped homologs score far above the decoys
-        assert min(g["sc"][0], g["sc"][2]) > max(g["sc"][1], g["sc"][3]) + 50.0, g["sc"]
+        # the clipped homologs (one keeps most of the model, one a short
+        # clipped piece) score above the decoys
+        assert min(g["sc"][0], g["sc"][2]) > max(g["sc"][1], g["sc"][3]) + 10.0, g["sc"]
