@@ -391,6 +391,20 @@ int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int trunca
  * Library-level controls and introspection (no counterpart in the reference).
  */
 int  infx_gpu_available(char *errbuf);                 /* 1 if a gfx9 HIP device can be used */
+/* FastCYKScan (same arguments, results bit-identical) that allocates no
+ * device memory: no hipMalloc, hipFree or allocating library call.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  Every device buffer of the call comes from one fixed 256 MiB array
+ * per GPU that the library holds until the process ends; a model and target
+ * whose buffers do not fit (profile, residues, chunk outputs, one DP slot and
+ * a minimal candidate buffer) are refused with eslERANGE and a message, and
+ * a scan whose candidates overflow the rest also returns eslERANGE.  Calls
+ * share the array: the library serialises them on a lock, and work issued
+ * into the array by other means would race with them. */
+int infx_FastCYKScanNoAlloc(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                            float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                            int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc);
 /* Keep a device copy of a target's residues dsq[1..L] (checked once here).
  * A later scan of the same buffer (the same dsq pointer, i0..j0 inside
  * 1..L) reads that copy in place instead of uploading its range, so a

@@ -245,6 +245,7 @@ const uint8_t *infx_pinned(const ESL_DSQ *dsq, int64_t x0, int64_t x1)
 int DevBuf::reserve(size_t n)
 {
   if (n <= cap) return eslOK;
+  if (borrowed) return eslERANGE;
   if (ptr) (void) hipFree(ptr);   /* contents are not kept; a failed free leaves nothing to undo */
   ptr = NULL; cap = 0;
   size_t want = n + n / 4 + 256;
@@ -252,7 +253,16 @@ int DevBuf::reserve(size_t n)
   cap = want;
   return eslOK;
 }
-void DevBuf::release() { if (ptr) (void) hipFree(ptr); ptr = NULL; cap = 0; }
+void DevBuf::release() { if (ptr && !borrowed) (void) hipFree(ptr); ptr = NULL; cap = 0; borrowed = false; }
+
+int Arena::give(DevBuf &b, size_t n)
+{
+  const size_t a = (n + 255) / 256 * 256;
+  if (a > left()) return eslERANGE;
+  b.ptr = base + used; b.cap = a; b.borrowed = true;
+  used += a;
+  return eslOK;
+}
 
 struct StdProfile {
   int qdbidx, semi, W;
@@ -532,7 +542,8 @@ static bool same_profile(const StdProfile &a, const StdProfile &b)
  * own (a chunk's DP never starts before its window), in one launch. */
 static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,
                     const int64_t *wi, const int64_t *wj, int nwin, float cutoff, CM_TOPHITS *hitlist, int do_null3,
-                    float env_cutoff, int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+                    float env_cutoff, int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc,
+                    Arena *ar = nullptr)
 {
   int status;
   auto t_start = std::chrono::steady_clock::now();
@@ -570,12 +581,50 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   StdProfile fresh;
   if ((status = build_std_profile(cm, smx, qdbidx, semi, W, fresh, errbuf)) != eslOK) return status;
 
+  /* DP slot of one wavefront: guard | deck | guard | rows[4] (ring over j) |
+   * guard | vsc | global accumulators (rows j, j+1; 4-byte cells) */
+  auto slot_layout = [&](int R, int RB, int gacc_, int acc_n_, size_t *deck_off_, size_t *row_off_, size_t *vsc_off_, size_t *acc_off_) {
+    const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
+    *deck_off_ = G * 4;
+    *row_off_  = *deck_off_ + ((size_t) RB + G) * 4;
+    *vsc_off_  = *row_off_ + ((size_t) 4 * R + G) * 4;
+    *acc_off_  = ((*vsc_off_ + (size_t) cm->M * 4) + 255) / 256 * 256;
+    return ((*acc_off_ + (gacc_ ? (size_t) acc_n_ * 8 : 0)) + 255) / 256 * 256;
+  };
+  auto r256 = [](size_t n) { return (n + 255) / 256 * 256; };
+  /* the no-allocation path: everything must fit the caller's fixed region
+   * (profile, residues, parameters, chunk outputs, one DP slot, a minimal
+   * candidate buffer), checked before any device work */
+  const int64_t ar_cand_min = ESL_MAX((int64_t) 4096, ESL_MIN((int64_t) 4 * L, (int64_t) 1 << 20));
+  if (ar) {
+    size_t d_, r_, v_, a_;
+    const int64_t nch_max = L / ESL_MAX((int64_t) 2 * W, (int64_t) 64) + nwin + 1;
+    const size_t need = r256(fresh.st.size() * sizeof(DState)) + r256(fresh.esc.size() * 4) + r256(fresh.rtmpl.size() * 4 + 4) +
+                        r256(fresh.dtmpl.size() * 4 + 4) + (semi != SEMI_CYK ? r256(INFX_LOGSUM_PAD * 4) : 0) +
+                        (pinned ? 256 : r256((size_t) Lspan + 128)) + r256(sizeof(ScanArgs)) + 256 +
+                        r256(ESL_MAX((size_t) nch_max * 24 + 64, r256((size_t) nch_max * 4) + (size_t) nch_max * 16 + 64)) +
+                        (nwin > 1 ? r256((size_t) nch_max * 24) : 0) +
+                        slot_layout(fresh.R, fresh.RB, 0, 0, &d_, &r_, &v_, &a_) + r256((size_t) ar_cand_min * sizeof(Cand));
+    if (need > ar->left())
+      return infx_fail(eslERANGE, errbuf, "%s: this model and target need %zu bytes of device scratch; the no-allocation scratch holds %zu",
+                       fname, need, ar->left());
+  }
+
   if ((status = infx_dev_init(errbuf)) != eslOK) return status;
-  if (!smx->dev) smx->dev = new SmxDev();
-  SmxDev *D = (SmxDev *) smx->dev;
+  SmxDev Dloc;           /* the no-allocation path's buffers (borrowed from the arena) */
+  StdProfile Ploc;
+  SmxDev *D = NULL;
   StdProfile *P = NULL;
-  for (auto *p : D->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->W == W) P = p;
-  if (!P) { P = new StdProfile(); D->prof.push_back(P); }
+  if (ar) { D = &Dloc; P = &Ploc; }
+  else {
+    if (!smx->dev) smx->dev = new SmxDev();
+    D = (SmxDev *) smx->dev;
+    for (auto *p : D->prof) if (p->qdbidx == qdbidx && p->semi == semi && p->W == W) P = p;
+    if (!P) { P = new StdProfile(); D->prof.push_back(P); }
+  }
+  if (ar && (ar->give(P->d_st, fresh.st.size() * sizeof(DState)) != eslOK || ar->give(P->d_esc, fresh.esc.size() * 4) != eslOK ||
+             ar->give(P->d_rtmpl, fresh.rtmpl.size() * 4 + 4) != eslOK || ar->give(P->d_dtmpl, fresh.dtmpl.size() * 4 + 4) != eslOK))
+    return infx_fail(eslERANGE, errbuf, "%s: profile does not fit the no-allocation scratch", fname);
   if (!P->uploaded || !same_profile(*P, fresh)) {
     P->st = fresh.st; P->esc = fresh.esc; P->rtmpl = fresh.rtmpl; P->dtmpl = fresh.dtmpl;
     P->R = fresh.R; P->RB = fresh.RB; P->root_dn = fresh.root_dn; P->root_dx = fresh.root_dx; P->root_cnum = fresh.root_cnum;
@@ -592,12 +641,14 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   }
   /* device tables carry INFX_LOGSUM_PAD entries, the ones past the
    * reference's 23000 zero */
-  if (semi == SEMI_FINSIDE && D->d_lsum_f.cap == 0) {
+  if (ar && semi == SEMI_FINSIDE && ar->give(D->d_lsum_f, INFX_LOGSUM_PAD * 4) != eslOK) return infx_fail(eslERANGE, errbuf, "%s: scratch full", fname);
+  if (ar && semi == SEMI_IINSIDE && ar->give(D->d_lsum_i, INFX_LOGSUM_PAD * 4) != eslOK) return infx_fail(eslERANGE, errbuf, "%s: scratch full", fname);
+  if (semi == SEMI_FINSIDE && (ar || D->d_lsum_f.cap == 0)) {
     std::vector<float> t(INFX_LOGSUM_PAD, 0.f);
     memcpy(t.data(), infx_flogsum_table(), LOGSUM_TBL * 4);
     if ((status = upload(D->d_lsum_f, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
   }
-  if (semi == SEMI_IINSIDE && D->d_lsum_i.cap == 0) {
+  if (semi == SEMI_IINSIDE && (ar || D->d_lsum_i.cap == 0)) {
     std::vector<int> t(INFX_LOGSUM_PAD, 0);
     memcpy(t.data(), infx_ilogsum_table(), LOGSUM_TBL * 4);
     if ((status = upload(D->d_lsum_i, t.data(), t.size() * 4)) != eslOK) return infx_fail(status, errbuf, "upload failed");
@@ -673,20 +724,30 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     for (int w = 0; w < nwin; w++)
       for (int64_t a = wi[w]; a <= wj[w]; a += chunk) { ctab.push_back(a); ctab.push_back(ESL_MIN(wj[w], a + chunk - 1)); ctab.push_back(wi[w]); }
   }
-  /* slot: guard | deck | guard | rows[4] (ring over j) | guard | vsc  (4-byte cells) */
-  const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
-  const size_t deck_off = G * 4;
-  const size_t row_off  = deck_off + ((size_t) P->RB + G) * 4;
-  const size_t vsc_off  = row_off + ((size_t) 4 * P->R + G) * 4;
-  const size_t acc_off  = ((vsc_off + (size_t) cm->M * 4) + 255) / 256 * 256;   /* global accumulators: rows j, j+1 */
-  const size_t slot_bytes = ((acc_off + (gacc ? (size_t) acc_n * 8 : 0)) + 255) / 256 * 256;
-  size_t mem_free = 0, mem_total = 0;
-  if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
-  const size_t scratch_budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
-                                + D->d_scratch.cap;
+  size_t deck_off, row_off, vsc_off, acc_off;
+  const size_t slot_bytes = slot_layout(P->R, P->RB, gacc, acc_n, &deck_off, &row_off, &vsc_off, &acc_off);
+  const size_t chunk_out_bytes = ESL_MAX((size_t) nchunks * 24 + 64, ((((size_t) nchunks * 4 + 255) / 256) * 256) + (size_t) nchunks * 16 + 64);
+  size_t scratch_budget;
+  if (ar) {
+    /* the arena's rest, less the fixed buffers and a minimal candidate buffer */
+    const size_t fixed = (pinned ? 256 : r256((size_t) Lspan + 128)) + r256(chunk_out_bytes) + 256 + r256(sizeof(ScanArgs)) +
+                         (nwin > 1 ? r256(ctab.size() * sizeof(int64_t)) : 0) + r256((size_t) ar_cand_min * sizeof(Cand));
+    scratch_budget = ar->left() > fixed ? ar->left() - fixed : 0;
+  } else {
+    size_t mem_free = 0, mem_total = 0;
+    if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
+    scratch_budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
+                     + D->d_scratch.cap;
+  }
   int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
-  if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+  if (nslots < 1) return infx_fail(ar ? eslERANGE : eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+  if (ar && (ar->give(D->d_dsq, pinned ? 64 : (size_t) Lspan + 128) != eslOK || ar->give(D->d_scratch, slot_bytes * nslots) != eslOK ||
+             ar->give(D->d_chunk, chunk_out_bytes) != eslOK || ar->give(D->d_ncand, 64) != eslOK ||
+             ar->give(D->d_args, sizeof(ScanArgs)) != eslOK ||
+             (nwin > 1 && ar->give(D->d_ctab, ctab.size() * sizeof(int64_t)) != eslOK) ||
+             ar->give(D->d_cand, ar->left() / 256 * 256) != eslOK))
+    return infx_fail(eslERANGE, errbuf, "%s: buffers do not fit the no-allocation scratch", fname);
 
   if ((status = D->d_dsq.reserve(pinned ? 64 : (size_t) Lspan + 128)) != eslOK ||
       (status = D->d_scratch.reserve(slot_bytes * nslots)) != eslOK ||
@@ -698,6 +759,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
    * exact size */
   int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 4 * L, ((int64_t) 8 << 30) / (int64_t) sizeof(Cand)));
   cand_cap = ESL_MAX(cand_cap, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
+  if (ar) cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   if ((status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   /* target residues i0..j0 behind and before 64 zero bytes (residue codes
@@ -775,7 +837,10 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     if (hipEventElapsedTime(&ms, g_ev0, g_ev1) != hipSuccess) { (void) hipGetLastError(); ms = 0; }
     kernel_ms_total += ms; launches++;
     if ((int64_t) ncand <= cand_cap) break;
-    if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: candidate buffer allocation failed", fname);
+    if ((status = D->d_cand.reserve((size_t) ncand * sizeof(Cand) + 4096)) != eslOK)
+      return infx_fail(ar ? eslERANGE : eslEMEM, errbuf, ar ? "%s: %llu candidate cells do not fit the no-allocation scratch (room for %lld)"
+                                                            : "%s: candidate buffer allocation failed (%llu candidates, %lld allocated)",
+                       fname, (unsigned long long) ncand, (long long) cand_cap);
     cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
     A.cand = (Cand *) D->d_cand.ptr; A.cand_cap = cand_cap;
   }
@@ -902,6 +967,36 @@ extern "C" int FastFInsideScan(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbi
   return std_scan(SEMI_FINSIDE, "FastFInsideScan", cm, errbuf, smx, qdbidx, dsq, &i0, &j0, 1, cutoff, hitlist, do_null3, env_cutoff,
                   ret_envi, ret_envj, ret_vsc, ret_sc);
 }
+/* FastCYKScan without device allocation: every device buffer of the call
+ * (profile, residues, parameters, DP slots, candidates, chunk outputs) is
+ * carved from the library's fixed scratch array on the current device, so
+ * the call makes no hipMalloc / hipFree.  Calls are serialised on a mutex:
+ * they share the array, which keeps the last call's contents (the counters a
+ * launch needs zeroed are cleared by the call).  A model and target whose
+ * buffers do not fit even one DP slot are refused with eslERANGE before any
+ * device work; the DP slots in use (and so the parallelism) are what the
+ * array holds after the fixed buffers. */
+static std::mutex g_noalloc_mu;
+extern "C" int infx_FastCYKScanNoAlloc(CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq, int64_t i0, int64_t j0,
+                                       float cutoff, CM_TOPHITS *hitlist, int do_null3, float env_cutoff,
+                                       int64_t *ret_envi, int64_t *ret_envj, float **ret_vsc, float *ret_sc)
+{
+  int status;
+  Arena ar;
+  void *base = NULL;
+  size_t bytes = 0;
+  if ((status = infx_dev_init(errbuf)) == eslOK && (status = infx_noalloc_scratch(&base, &bytes, errbuf)) != eslOK) return status;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (status == eslOK) { ar.base = (char *) base; ar.cap = bytes; }
+  else {
+    /* no device: the size check still runs (against the array's size), then
+     * the scan reports the missing device */
+    ar.base = NULL; ar.cap = INFX_NOALLOC_BYTES;
+  }
+  return std_scan(SEMI_CYK, "FastCYKScan", cm, errbuf, smx, qdbidx, dsq, &i0, &j0, 1, cutoff, hitlist, do_null3, env_cutoff,
+                  ret_envi, ret_envj, ret_vsc, ret_sc, &ar);
+}
+
 /* Many windows of one target in one launch (no counterpart in the
  * reference: the windows' hits are those of one standard scan per window). */
 extern "C" int infx_FastScanWindows(int semi, CM_t *cm, char *errbuf, CM_SCAN_MX *smx, int qdbidx, ESL_DSQ *dsq,

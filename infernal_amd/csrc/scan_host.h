@@ -10,9 +10,24 @@
 struct DevBuf {
   void  *ptr = nullptr;
   size_t cap = 0;
-  int  reserve(size_t n);   /* grow (contents not kept) */
+  bool   borrowed = false;  /* memory lent by an arena: never grown or freed here */
+  int  reserve(size_t n);   /* grow (contents not kept); a borrowed buffer cannot grow: eslERANGE */
   void release();
 };
+
+/* A fixed device region that one call's buffers are carved from (the
+ * no-allocation scanner's static scratch array): bump allocation, 256-byte
+ * aligned, nothing freed. */
+struct Arena {
+  char  *base = nullptr;
+  size_t cap = 0, used = 0;
+  size_t left() const { return cap > used ? cap - used : 0; }
+  int give(DevBuf &b, size_t n);
+};
+
+/* The no-allocation scanner's scratch array on the current device: its
+ * device address (looked up once per device) and size. */
+int infx_noalloc_scratch(void **ret_ptr, size_t *ret_bytes, char *errbuf);
 
 extern "C" int  infx_dev_init(char *errbuf);
 extern "C" void infx_dev_release(void *dev);

@@ -32,6 +32,7 @@
 
 #define INFX_LOGSUM_N 23000   /* entries of the FLogsum / ILogsum tables */
 #define INFX_LOGSUM_PAD 23008 /* device copies: entries 23000.. are 0 (FLogsum's 'return max' index) */
+#define INFX_NOALLOC_BYTES ((size_t) 256 << 20)   /* the no-allocation scanner's fixed scratch per GPU */
 
 enum { KIND_GENERIC = 0, KIND_IL = 1, KIND_B = 2, KIND_BEGL = 3, KIND_SKIP = 4 };
 enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };

@@ -42,6 +42,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include "scan_kernels.h"
 
 #define IMPF        (-1e36f)
@@ -979,4 +980,40 @@ extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hip
   e = hipLaunchKernel(f, grid, block, args, lds, stream);
   if (e != hipSuccess) return e;
   return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scratch of the no-allocation scanner (infx_FastCYKScanNoAlloc): one fixed
+ * device array per GPU, held until the process ends.  In host code the name
+ * `scratch` is only a stand-in: its device address comes from
+ * hipGetSymbolAddress, looked up once per device with that device current,
+ * and only that address is passed to kernels and copies. */
+static __device__ __align__(256) unsigned char scratch[INFX_NOALLOC_BYTES];
+
+#include <mutex>
+#define INFX_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[INFX_MAX_DEVICES];
+
+int infx_noalloc_scratch(void **ret_ptr, size_t *ret_bytes, char *errbuf)
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= INFX_MAX_DEVICES) {
+    (void) hipGetLastError();
+    if (errbuf) snprintf(errbuf, 128, "no-allocation scratch: no current HIP device");
+    return 100;   /* eslENODEVICE */
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = NULL;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      if (errbuf) snprintf(errbuf, 128, "no-allocation scratch: hipGetSymbolAddress failed (%s)", hipGetErrorString(e));
+      return 12;   /* eslESYS */
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *ret_ptr = g_scratch_ptr[dev];
+  *ret_bytes = INFX_NOALLOC_BYTES;
+  return 0;
 }
