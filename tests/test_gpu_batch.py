@@ -54,3 +54,63 @@ def test_batch_equals_per_model_windows(algo):
         assert hits_equal(g, w), (names[m], describe_hit_diff(g, w))
         assert np.float32(got[m]["sc"]).view(np.uint32) == np.float32(want["sc"]).view(np.uint32), (names[m], got[m]["sc"], want["sc"])
     assert sum(len(x["hits"]) for x in got) > 0
+
+
+CFG4_MODELS = ["synth_cloverleaf76.cm", "tRNA.c.cm", "synth_multiloop350.cm", "Cobalamin.c.cm", "synth_nested1500.cm"]
+
+
+def _cfg4_windows(cms, n_per, L, seed):
+    """n_per windows of 1.2 W per model at seeded, per-model disjoint
+    positions of a target of L residues (models' windows overlap freely)."""
+    rng = np.random.default_rng(seed)
+    wins = []
+    for m, cm in enumerate(cms):
+        wl = int(1.2 * cm.W)
+        starts = np.sort(rng.choice(np.arange(1, L - wl, wl + 1), n_per, replace=False))
+        wins += [(int(a), int(a) + wl - 1, m) for a in starts]
+    return wins
+
+
+@pytest.mark.parametrize("algo", ["cyk", "iinside"])
+def test_batch_against_oracle_model_sizes(algo):
+    """The many-models workload's shape against the oracle: five models of
+    76 to 1,500 consensus nt (W 89 to 1,522) in one launch, windows of 1.2 W,
+    a target of random background with each model's emitted homologs planted;
+    every model's hits and best score equal the oracle's scans of its windows
+    one by one (hits of the windows pooled, as a batched call returns them)."""
+    import concurrent.futures as cf
+    import oracle_bind as ob
+    from infernal_amd import synth
+    cms = [load_cm(n) for n in CFG4_MODELS]
+    frags = []
+    for k, n in enumerate(CFG4_MODELS):
+        frags += synth.homologs(ia.CM.read(os.path.join(CMDIR, n)), 2, seed=700 + k)
+    L = 60000
+    bg = synth.markov2_background(L, gc=0.45, seed=707)
+    codes, _ = synth.plant(bg, frags, seed=709)
+    dsq = ia.dsq_from_codes(codes)
+    n_per = 3 if algo == "cyk" else 2
+    wins = _cfg4_windows(cms, n_per, L, seed=711)
+    if algo == "iinside":   # the 1,500-nt model's Inside takes the oracle ~15 s per window: one
+        wins = [w for w in wins if w[2] != 4 or w[0] == min(x[0] for x in wins if x[2] == 4)]
+    qdb = 1 if algo == "cyk" else 2
+    cutoffs = [0.0] * len(cms)
+    print("case batch-vs-oracle", algo, [(w[2], w[1] - w[0] + 1) for w in wins], flush=True)
+    got = ia.ScanBatch(algo, cms, qdb, dsq, wins, cutoffs)
+    oalgo = ob.ORACLE_CYK if algo == "cyk" else ob.ORACLE_IINSIDE
+    for cm in cms:
+        if algo == "cyk":
+            cm.c.search_opts &= ~ia.CM_SEARCH_INSIDE
+        else:
+            cm.c.search_opts |= ia.CM_SEARCH_INSIDE
+    with cf.ThreadPoolExecutor(8) as ex:
+        outs = list(ex.map(lambda w: ob.scan(cms[w[2]], oalgo, qdb, dsq, w[0], w[1], 0.0), wins))
+    key = lambda h: (h[0], h[1])
+    for m, name in enumerate(CFG4_MODELS):
+        mine = [o for w, o in zip(wins, outs) if w[2] == m]
+        want = sorted([h for o in mine for h in o["hits"]], key=key)
+        g = sorted(got[m]["hits"], key=key)
+        assert hits_equal(g, want), (name, describe_hit_diff(g, want))
+        best = max(o["sc"] for o in mine)
+        assert np.float32(got[m]["sc"]).view(np.uint32) == np.float32(best).view(np.uint32), (name, got[m]["sc"], best)
+    assert sum(len(x["hits"]) for x in got) > 0
