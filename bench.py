@@ -234,11 +234,13 @@ def batch_leg(args, ia, synth):
 
 def trunc_leg(args, ia, synth, threads):
     """Truncated scanning with the 1,500-nt model: --trunc-windows windows of
-    200-2,000 nt (uniform), half an emitted homolog clipped by the window at
-    a random offset, half random ACGU decoys.  RefTrCYKScan (QDB1) and
-    RefITrInsideScan (QDB2) in the 5'-and-3' truncation pass (J, L, R and T
-    modes filled), all windows in one launch each; the oracle on the host
-    scans a sample of the same windows."""
+    200 to --trunc-lmax nt (uniform), half an emitted homolog clipped by the
+    window at a random offset, half random ACGU decoys.  RefTrCYKScan (QDB1)
+    and RefITrInsideScan (QDB2) in the 5'-and-3' truncation pass (J, L, R and
+    T modes filled), --trunc-batch windows per launch (0: all in one launch,
+    one workgroup per window); the oracle on the host scans a sample of the
+    windows (one per thread, of at most --trunc-cpu-lmax nt) with the same
+    scanner, so both rates are banded cells per second."""
     path = os.path.join(ROOT, "tests", "golden", "cm", "synth_nested1500.cm")
     cm = ia.CM.read(path)
     cm.configure(local=True, trunc=True, scanmx=True, trscanmx=True)
@@ -249,11 +251,10 @@ def trunc_leg(args, ia, synth, threads):
     out = {"cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), cm.clen, cm.M, cm.W),
            "windows": len(wins), "window_nt": [200, args.trunc_lmax], "residues": int(len(codes)), "pass": "5P_AND_3P_FORCE",
            "note": "cells are banded (v, j, d) cells; each is computed in J, L and R mode (T too for bifurcations)"}
-    # launches of --trunc-batch windows of similar length (one wavefront per
-    # window: a launch lasts as long as its longest window)
     order = sorted(range(len(wins)), key=lambda k: wins[k][1] - wins[k][0])
-    batches = [order[a:a + args.trunc_batch] for a in range(0, len(order), args.trunc_batch)]
-    out["launch_windows"] = args.trunc_batch
+    nb = args.trunc_batch if args.trunc_batch > 0 else len(order)
+    batches = [order[a:a + nb] for a in range(0, len(order), nb)]
+    out["launch_windows"] = nb
     for algo, q in (("cyk", 1), ("iinside", 2)):
         ia.TrScanWindows(algo, cm, q, pidx, dsq, [wins[order[0]]], 0.0)    # first-use set-up
         wall = kern = cells = 0.0
@@ -282,24 +283,21 @@ def trunc_leg(args, ia, synth, threads):
         import oracle_bind as ob
         ob.load()
         rng = np.random.default_rng(77)
-        sample = [wins[k] for k in rng.choice(len(wins), min(threads, len(wins)), replace=False)]
-
-        def one(w):
-            ob.scan(cm, ob.ORACLE_TRCYK, 1, dsq, int(w[0]), int(w[1]), 1e30, want_rows=False, pass_idx=pidx)
-
-        t0 = time.perf_counter()
-        with cf.ThreadPoolExecutor(len(sample)) as ex:
-            list(ex.map(one, sample))
-        cw = time.perf_counter() - t0
-        log("truncated CPU sample: %d windows in %.1f s" % (len(sample), cw))
-        cells = sum(_cells(cm, 1, int(w[0]), int(w[1]), truncated=True) for w in sample)
-        t1 = time.perf_counter()
-        one(sample[0])
-        c1 = time.perf_counter() - t1
-        out["cpu"] = {"cyk_gcells_per_s": round(cells / cw / 1e9, 4), "cores": len(sample),
-                      "cyk_1core_gcells_per_s": round(_cells(cm, 1, int(sample[0][0]), int(sample[0][1]), truncated=True) / c1 / 1e9, 4),
-                      "sample": "oracle RefTrCYKScan on %d of the windows in %d threads, %.1f s" % (len(sample), len(sample), cw)}
-        out["gpu_over_cpu_cyk"] = round(out["cyk"]["gpu_gcells_per_s"] / out["cpu"]["cyk_gcells_per_s"], 1)
+        short = [k for k in range(len(wins)) if wins[k][1] - wins[k][0] + 1 <= args.trunc_cpu_lmax]
+        sample = [wins[k] for k in rng.choice(short, min(threads, len(short)), replace=False)]
+        out["cpu"] = {"cores": len(sample), "kind": "port",
+                      "sample": "oracle, %d of the windows of at most %d nt, one per thread" % (len(sample), args.trunc_cpu_lmax)}
+        for algo, oalgo, q in (("cyk", ob.ORACLE_TRCYK, 1), ("iinside", ob.ORACLE_TRIINSIDE, 2)):
+            if algo == "iinside":
+                cm.c.search_opts |= ia.CM_SEARCH_INSIDE
+            t0 = time.perf_counter()
+            with cf.ThreadPoolExecutor(len(sample)) as ex:
+                list(ex.map(lambda w: ob.scan(cm, oalgo, q, dsq, int(w[0]), int(w[1]), 1e30, want_rows=False, pass_idx=pidx), sample))
+            cw = time.perf_counter() - t0
+            cells = sum(_cells(cm, q, int(w[0]), int(w[1]), truncated=True) for w in sample)
+            log("truncated CPU %s sample: %d windows in %.1f s" % (algo, len(sample), cw))
+            out["cpu"][algo + "_gcells_per_s"] = round(cells / cw / 1e9, 4)
+            out["gpu_over_cpu_" + algo] = round(out[algo]["gpu_gcells_per_s"] / out["cpu"][algo + "_gcells_per_s"], 1)
     return out
 
 
@@ -450,7 +448,8 @@ def main():
                          "run in a child process); 0 = skip")
     ap.add_argument("--trunc-lmax", type=int, default=1000,
                     help="longest truncated-scan window")
-    ap.add_argument("--trunc-batch", type=int, default=16, help="windows per truncated-scan launch")
+    ap.add_argument("--trunc-batch", type=int, default=0, help="windows per truncated-scan launch (0: all in one launch)")
+    ap.add_argument("--trunc-cpu-lmax", type=int, default=500, help="longest window of the truncated leg's CPU sample")
     ap.add_argument("--trunc-only", action="store_true", help="run only the truncated-scan leg and print its JSON")
     ap.add_argument("--batch-windows", type=int, default=0,
                     help="windows per model of the many-models leg (one launch for all models vs one per model); 0 = skip")
@@ -652,7 +651,8 @@ def main():
         # memory are released when it ends
         import subprocess
         cmd = [sys.executable, "-u", os.path.abspath(__file__), "--trunc-only", "--trunc-windows", str(args.trunc_windows),
-               "--trunc-lmax", str(args.trunc_lmax), "--trunc-batch", str(args.trunc_batch)]
+               "--trunc-lmax", str(args.trunc_lmax), "--trunc-batch", str(args.trunc_batch),
+               "--trunc-cpu-lmax", str(args.trunc_cpu_lmax)]
         if args.no_cpu:
             cmd.append("--no-cpu")
         if args.cpu_threads:

@@ -391,6 +391,12 @@ int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int trunca
  * Library-level controls and introspection (no counterpart in the reference).
  */
 int  infx_gpu_available(char *errbuf);                 /* 1 if a gfx9 HIP device can be used */
+/* infx_FastScanBatch through a HIP graph (on != 0): the counter reset (a
+ * kernel) and the batched kernel are captured once and the graph is replayed
+ * by later calls whose launch shape and device buffers are the same; the
+ * parameters are uploaded before each replay.  Results are identical to the
+ * direct launch.  No counterpart in the reference. */
+void infx_set_batch_graph(int on);
 /* FastCYKScan (same arguments, results bit-identical) that allocates no
  * device memory: no hipMalloc, hipFree or allocating library call.  No
  * counterpart in the reference; it is for callers that cannot allocate
@@ -433,8 +439,8 @@ int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, int6
  * descriptors (one per state), up to pen_cap 24-byte truncated-begin entries,
  * the per-64-cell-block begin lists (offsets, then entries), each state's
  * band of stored d cells (lohi[2v], lohi[2v+1]; lo > hi: none) and
- * dims[0..6] = row cells, deck cells, T-row cells, emission entries, root
- * band max, begin entries, begin-list length.  Returns the descriptor count,
+ * dims[0..6] = row cells, deck-ring cells, T-row cells, emission entries,
+ * root band max, begin entries, begin-list length.  Returns the descriptor count,
  * or -status.  No counterpart in the reference: used to check the truncated
  * kernel's address arithmetic on the host. */
 int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, void *out_st, int st_cap,
@@ -445,6 +451,14 @@ int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, 
  * resources or nprof is outside 0..8.  No counterpart in the reference: lets
  * a CPU test (under AddressSanitizer) check the truncated matrix's teardown. */
 int infx_debug_attach_trdev(CM_t *cm, CM_TR_SCAN_MX *trsmx, int nprof);
+/* The row schedule of the truncated kernel for nw (1..8) wavefronts per
+ * window: each state's same-row dependency level is in the descriptors
+ * (infx_debug_tr_profile); out_sched[nlev*nw + 1] holds the item offsets of
+ * (level, wavefront) and out_items the items as (state, first block, end
+ * block) triples.  Returns the item count, or -status; *ret_nlev = levels.
+ * No counterpart in the reference: used to check the schedule on the host. */
+int infx_debug_tr_schedule(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, int nw, int want_vsc,
+                           int *out_sched, int sched_cap, int *out_items, int items_cap, int *ret_nlev, char *errbuf);
 /* Query-dependent bands and W from the model's transition probabilities
  * (local ends off, standard local begins on).  Fills qdbinfo->dmin1/dmax1
  * (beta1) and dmin2/dmax2 (beta2) for every state and *ret_W (beta_W).

@@ -159,6 +159,12 @@ def unpin_target():
     _lib.infx_target_Unpin()
 
 
+def set_batch_graph(on):
+    """infx_set_batch_graph: batched launches (ScanBatch) through a captured
+    HIP graph, replayed while the launch shape stays the same."""
+    _lib.infx_set_batch_graph(1 if on else 0)
+
+
 def last_call_timing():
     w, k, c, n = C.c_double(), C.c_double(), C.c_int64(), C.c_int()
     _lib.infx_last_call_timing(C.byref(w), C.byref(k), C.byref(c), C.byref(n))

@@ -21,6 +21,7 @@ extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int g
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc);
 extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
 extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n);
+extern "C" hipError_t infx_launch_zero_u64(unsigned long long *p, hipStream_t stream);
 extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a, int acc_n);
 
 /* ------------------------------------------------------------------------ */
@@ -131,6 +132,7 @@ static hipEvent_t  g_ev0, g_ev1;
 static int  g_ncu = 256;
 static size_t g_lds_max = 65536;   /* LDS bytes one workgroup may allocate */
 static int  g_chunk_override = 0, g_wpb_override = 0;
+static int  g_batch_graph = 0;     /* batched launches through a captured graph */
 
 static thread_local double  t_wall_ms = 0, t_kernel_ms = 0;
 static thread_local int64_t t_cells = 0;
@@ -188,6 +190,7 @@ extern "C" void infx_set_launch_params(int chunk_len, int waves_per_block)
   g_chunk_override = chunk_len > 0 ? chunk_len : 0;
   g_wpb_override   = (waves_per_block >= 1 && waves_per_block <= 16) ? waves_per_block : 0;
 }
+extern "C" void infx_set_batch_graph(int on) { g_batch_graph = on ? 1 : 0; }
 int infx_chunk_override(void) { return g_chunk_override; }
 int infx_wpb_override(void) { return g_wpb_override; }
 
@@ -281,7 +284,14 @@ struct StdProfile {
 struct SmxDev {
   std::vector<StdProfile *> prof;
   DevBuf d_dsq, d_scratch, d_cand, d_chunk, d_ncand, d_lsum_f, d_lsum_i, d_args, d_ctab, d_btab;
+  /* the batched launch as a graph (counter reset + kernel), reused while its
+   * launch shape and parameter buffers stay the same */
+  hipGraph_t graph = NULL;
+  hipGraphExec_t gexec = NULL;
+  size_t gkey[6] = {0, 0, 0, 0, 0, 0};
   ~SmxDev() {
+    if (gexec) (void) hipGraphExecDestroy(gexec);
+    if (graph) (void) hipGraphDestroy(graph);
     for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); delete p; }
     d_dsq.release(); d_scratch.release(); d_cand.release(); d_chunk.release(); d_ncand.release(); d_args.release(); d_ctab.release();
     d_btab.release(); d_lsum_f.release(); d_lsum_i.release();
@@ -1220,11 +1230,41 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     if ((status = upload(D->d_btab, args.data(), args.size() * sizeof(ScanArgs))) != eslOK)
       return infx_fail(eslEMEM, errbuf, "%s: parameter upload failed", fname);
     Am.args_tab = (const ScanArgs *) D->d_btab.ptr;
-    if (hipMemsetAsync(Am.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
-    (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
     if (upload(D->d_args, &Am, sizeof(Am)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
-    hipError_t e = infx_launch_std_batch(semi, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_max);
-    (void) hipEventRecord(g_ev1, g_stream);
+    hipError_t e = hipSuccess;
+    if (g_batch_graph) {
+      /* counter reset and kernel as one graph; the kernel reads its
+       * parameters from device memory (uploaded above), so the graph is
+       * reused while the launch shape and the buffers it names stay put */
+      const size_t key[6] = {(size_t) semi, (size_t) grid.x, (size_t) block.x, (size_t) acc_max, (size_t) D->d_args.ptr, (size_t) Am.ncand};
+      if (!D->gexec || memcmp(key, D->gkey, sizeof(key)) != 0) {
+        if (D->gexec) { (void) hipGraphExecDestroy(D->gexec); D->gexec = NULL; }
+        if (D->graph) { (void) hipGraphDestroy(D->graph); D->graph = NULL; }
+        hipGraph_t g = NULL;
+        if ((e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal)) == hipSuccess) {
+          hipError_t e1 = infx_launch_zero_u64(Am.ncand, g_stream);
+          hipError_t e2 = (e1 == hipSuccess) ? infx_launch_std_batch(semi, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_max) : e1;
+          e = hipStreamEndCapture(g_stream, &g);
+          if (e == hipSuccess && e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&D->gexec, g, NULL, NULL, 0);
+        if (e != hipSuccess) {
+          if (g) (void) hipGraphDestroy(g);
+          D->gexec = NULL;
+          return infx_fail(eslESYS, errbuf, "%s: graph capture failed: %s", fname, hipGetErrorString(e));
+        }
+        D->graph = g;
+        memcpy(D->gkey, key, sizeof(key));
+      }
+      (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
+      e = hipGraphLaunch(D->gexec, g_stream);
+      (void) hipEventRecord(g_ev1, g_stream);
+    } else {
+      if (hipMemsetAsync(Am.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
+      (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
+      e = infx_launch_std_batch(semi, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_max);
+      (void) hipEventRecord(g_ev1, g_stream);
+    }
     if (e != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: kernel launch failed: %s", fname, hipGetErrorString(e));
     if (hipMemcpyAsync(&ncand, Am.ncand, sizeof(ncand), hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
         hipStreamSynchronize(g_stream) != hipSuccess)

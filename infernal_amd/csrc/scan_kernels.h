@@ -133,8 +133,16 @@ struct TState {
   int32_t kdx_y, kdx_w;
   uint32_t endsc;
   uint32_t tsc[6];
-  int32_t child_base[6];
-  int32_t pad[4];
+  int32_t child_base[6];   /* B: [0] BEGL deck base, [1] BEGR row base, [2] BEGL ring mask, [3] BEGL ring stride */
+  int32_t ring_mask;       /* BEGL: deck ring rows - 1 (a power of two minus one) */
+  int32_t ring_stride;     /* BEGL: cells per deck ring row */
+  int32_t level;           /* same-row dependency level (scheduling) */
+  int32_t pad;
+};
+
+/* one unit of a row's work: 64-cell blocks b0..b1-1 of state v */
+struct TrItem {
+  int32_t v, b0, b1;
 };
 
 /* a state with a finite truncated-begin penalty, in ascending order */
@@ -167,6 +175,9 @@ struct TrScanArgs {
   const int     *pen_blk;    /* [nblk + 1] offsets, then per 64-cell root block b the
                                 indices (ascending) of the begins whose band reaches d = 64b + 1 */
   int            pen_nblk;
+  const TrItem  *items;      /* a row's work, level by level; within a level, wave by wave */
+  const int     *sched;      /* [nlev * nw + 1]: items of (level l, wave w) are items[sched[l*nw+w] .. sched[l*nw+w+1]) */
+  int            nlev, nw;
   const TrWindow *win;       /* longest first */
   int            nwin;
   unsigned      *next_win;   /* work counter: wavefronts take windows in order */

@@ -982,6 +982,18 @@ extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hip
   return hipGetLastError();
 }
 
+/* one 64-bit counter set to zero by a kernel (a graph replays kernels
+ * reliably; a counter reset is a kernel node, not a memset node) */
+__global__ void zero_u64_kernel(unsigned long long *p)
+{
+  if (threadIdx.x == 0) *p = 0ull;
+}
+extern "C" hipError_t infx_launch_zero_u64(unsigned long long *p, hipStream_t stream)
+{
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, stream, p);
+  return hipGetLastError();
+}
+
 /* ------------------------------------------------------------------------ */
 /* Scratch of the no-allocation scanner (infx_FastCYKScanNoAlloc): one fixed
  * device array per GPU, held until the process ends.  In host code the name

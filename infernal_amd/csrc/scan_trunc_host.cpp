@@ -18,6 +18,7 @@
 #include "scan_host.h"
 
 extern "C" hipError_t infx_launch_tr(int semi, int want_vsc, dim3 grid, dim3 block, hipStream_t stream, const TrScanArgs *a);
+extern "C" int infx_tr_occupancy(int semi, int want_vsc, int nw);
 
 static uint32_t fb(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static int f2i_host(float f) { return (f >= -2147483648.0f && f < 2147483648.0f) ? (int) f : INT_MIN; }
@@ -28,17 +29,26 @@ struct TrProfile {
   std::vector<uint32_t> esc, rtmpl, dtmpl;
   std::vector<TrPen> pen;
   std::vector<int> pblk;     /* per root block: offsets, then begin indices */
-  int R = 0, RB = 0, RT = 0, root_dx = 0;
+  int R = 0, RB = 0, RT = 0, root_dx = 0, nlev = 0;
   uint32_t el = 0;
   DevBuf d_st, d_esc, d_rtmpl, d_dtmpl, d_pen, d_pblk;
   bool uploaded = false;
+  /* the row schedule for sched_nw wavefronts (per-state items if sched_vsc) */
+  int sched_nw = 0, sched_vsc = -1;
+  std::vector<int> sched;
+  std::vector<TrItem> items;
+  DevBuf d_sched, d_items;
 };
 
 struct TrDev {
   std::vector<TrProfile *> prof;
   DevBuf d_dsq, d_scratch, d_cand, d_ncand, d_win, d_wout, d_lsum;
   ~TrDev() {
-    for (auto *p : prof) { p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); p->d_pen.release(); p->d_pblk.release(); delete p; }
+    for (auto *p : prof) {
+      p->d_st.release(); p->d_esc.release(); p->d_rtmpl.release(); p->d_dtmpl.release(); p->d_pen.release(); p->d_pblk.release();
+      p->d_sched.release(); p->d_items.release();
+      delete p;
+    }
     d_dsq.release(); d_scratch.release(); d_cand.release(); d_ncand.release(); d_win.release(); d_wout.release(); d_lsum.release();
   }
 };
@@ -96,7 +106,9 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
         int kmin, kmax;
         if (banded) { kmin = ESL_MAX(0, d - ESL_MIN(dmax[w], W)); kmax = ESL_MIN(ESL_MIN(dmax[y], W), d); }
         else { kmin = 0; kmax = d; }
-        extend(y, d, d); extend(w, d, d);
+        if (kmin == 0) extend(w, d, d);    /* L mode: BEGL w alone, this row */
+        if (kmax == d) extend(y, d, d);    /* R mode: BEGR y alone */
+        extend(w, 0, 0);                   /* idle lanes of a block read cell 0 */
         if (kmin > kmax) continue;
         extend(y, kmin, kmax);
         extend(w, d - kmax, d - kmin);
@@ -108,24 +120,40 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
       for (int k = 0; k < cm->cnum[v]; k++) extend(cm->cfirst[v] + k, a, b);
     }
   }
+  /* deck ring rows of each BEGL state: its parent reads rows j-k for k up to
+   * the BEGR band (kdx_y), so a power of two above kdx_y rows suffices */
+  std::vector<int> ring(M, 0), parent_b(M, -1);
+  for (int v = 1; v < M; v++)
+    if (cm->sttype[v] == B_st) {
+      const int w = cm->cfirst[v], y = cm->cnum[v];
+      parent_b[w] = v;
+      const int kdy = banded ? ESL_MIN(dmax[y], W) : W;
+      int r = 1;
+      while (r < kdy + 1) r <<= 1;
+      ring[w] = r;
+    }
   std::vector<uint32_t> a0;
   tr_d0_init(cm, semi, a0);
   const uint32_t imp = (semi == TSEMI_IINSIDE) ? (uint32_t) (-INFTY) : fb((float) IMPOSSIBLE);
   const uint32_t zero = (semi == TSEMI_IINSIDE) ? 0u : fb(0.f);
-  std::vector<int> base(M, 0), tbase(M, 0);
+  std::vector<int> base(M, 0), tbase(M, 0), stride(M, 0);
   std::vector<uint32_t> rJ, rL, rR, dJ, dL, dR;
   for (int v = 1; v < M; v++) {
     if (lo[v] > hi[v]) continue;
     lo[v] = ESL_MAX(lo[v], 0); hi[v] = ESL_MIN(hi[v], W);
     if (lo[v] > hi[v]) continue;
     bool begl = (cm->stid[v] == BEGL_S);
+    if (begl && ring[v] == 0) return infx_fail(eslEINVAL, errbuf, "BEGL state %d has no bifurcation parent", v);
     std::vector<uint32_t> &tj = begl ? dJ : rJ, &tl = begl ? dL : rL, &tr = begl ? dR : rR;
     base[v] = (int) tj.size() - lo[v];
-    for (int d = lo[v]; d <= hi[v]; d++) {
-      tj.push_back(d == 0 ? a0[v] : imp);
-      tl.push_back((d == 0 && cm->sttype[v] == E_st) ? zero : imp);
-      tr.push_back((d == 0 && cm->sttype[v] == E_st) ? zero : imp);
-    }
+    stride[v] = hi[v] - lo[v] + 1;
+    for (int r = 0; r < (begl ? ring[v] : 1); r++)     /* a BEGL state: every ring row starts as the template */
+      for (int d = lo[v]; d <= hi[v]; d++) {
+        tj.push_back(d == 0 ? a0[v] : imp);
+        tl.push_back((d == 0 && cm->sttype[v] == E_st) ? zero : imp);
+        tr.push_back((d == 0 && cm->sttype[v] == E_st) ? zero : imp);
+      }
+    if ((int64_t) tj.size() > INT32_MAX / 4) return infx_fail(eslERANGE, errbuf, "truncated scan profile too large");
   }
   if (ret_lo) *ret_lo = lo;
   if (ret_hi) *ret_hi = hi;
@@ -175,14 +203,29 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
       s.endsc = fb(cm->endsc[v]);
       if (st != B_st && st != E_st) for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.tsc[k] = fb(cm->tsc[v][k]);
     }
+    if (cm->stid[v] == BEGL_S) { s.ring_mask = ring[v] - 1; s.ring_stride = stride[v]; }
     if (st == B_st) {
       int w = cm->cfirst[v], y = cm->cnum[v];
       s.child_base[0] = base[w]; s.child_base[1] = base[y];
+      s.child_base[2] = ring[w] - 1; s.child_base[3] = stride[w];
       s.banded = banded;
       if (banded) { s.kdx_y = ESL_MIN(dmax[y], W); s.kdx_w = ESL_MIN(dmax[w], W); }
     } else if (st != E_st) {
       for (int k = 0; k < cm->cnum[v] && k < 6; k++) s.child_base[k] = base[cm->cfirst[v] + k];
     }
+  }
+  /* same-row dependency levels: a state reads cells of the same row j from
+   * its children (B: BEGL w and BEGR y), so it comes after all of them */
+  P.nlev = 0;
+  for (int v = M - 1; v >= 1; v--) {
+    TState &s = P.st[v];
+    if (s.kind == KIND_SKIP || s.dx < 1) { s.level = -1; continue; }
+    int lv = 0;
+    auto dep = [&](int c) { if (c != v && c < M && P.st[c].level >= 0 && P.st[c].kind != KIND_SKIP && P.st[c].dx >= 1) lv = ESL_MAX(lv, P.st[c].level + 1); };
+    if (s.kind == KIND_B) { dep(cm->cfirst[v]); dep(cm->cnum[v]); }
+    else for (int k = 0; k < cm->cnum[v]; k++) dep(cm->cfirst[v] + k);
+    s.level = lv;
+    P.nlev = ESL_MAX(P.nlev, lv + 1);
   }
   P.pen.clear();
   const int local = (cm->flags & CMH_LOCAL_BEGIN) ? 1 : 0;
@@ -222,6 +265,60 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
   if (P.pen.empty()) { TrPen z; memset(&z, 0, sizeof(z)); P.pen.push_back(z); P.pen.pop_back(); }
   P.qdbidx = qdbidx; P.semi = semi; P.pty = pty; P.local = local;
   return eslOK;
+}
+
+/* A row's work for nw wavefronts: the states of each dependency level cut
+ * into items (a bifurcation or ordinary state: one item per 64-cell block of
+ * its steady-state band; an IL state, whose blocks chain, or any state when
+ * per-state maxima are wanted: one item), dealt largest first to the least
+ * loaded wavefront.  Costs are estimates in lane-steps: cells x child terms
+ * x modes; a bifurcation block its k range (plus the lanes' spread) x modes;
+ * an IL chain a few fixed-point sweeps per block. */
+static void build_tr_schedule(const CM_t *cm, TrProfile &P, int nw, int vsc, int fill_L, int fill_R, int fill_T)
+{
+  const int M = cm->M;
+  const int modes = 1 + fill_L + fill_R;
+  std::vector<std::vector<std::pair<double, TrItem>>> lev(P.nlev);
+  for (int v = M - 1; v >= 1; v--) {
+    const TState &s = P.st[v];
+    if (s.level < 0) continue;
+    const int nb = (s.dx + 63) / 64;
+    auto add = [&](int b0, int b1, double cost) { TrItem it; it.v = v; it.b0 = b0; it.b1 = b1; lev[s.level].push_back({cost, it}); };
+    if (s.kind == KIND_B) {
+      std::vector<double> c(nb);
+      for (int b = 0; b < nb; b++) {
+        const int d0 = 64 * b + 1, dl = ESL_MIN(d0 + 63, s.dx);
+        const int kb0 = s.banded ? ESL_MAX(0, d0 - s.kdx_w) : 0, kb1 = s.banded ? ESL_MIN(s.kdx_y, dl) : dl;
+        c[b] = 64.0 * (ESL_MAX(0, kb1 - kb0 + 1) + 8) * (modes + fill_T);
+      }
+      if (vsc) { double t = 0; for (double x : c) t += x; add(0, nb, t); }
+      else for (int b = 0; b < nb; b++) add(b, b + 1, c[b]);
+    } else {
+      const double per = 64.0 * (ESL_MAX(1, s.cnum) + 2) * modes * (s.kind == KIND_IL ? 3 : 1);
+      if (vsc || s.kind == KIND_IL) add(0, nb, per * nb);
+      else for (int b = 0; b < nb; b++) add(b, b + 1, per);
+    }
+  }
+  P.sched.assign((size_t) P.nlev * nw + 1, 0);
+  P.items.clear();
+  for (int l = 0; l < P.nlev; l++) {
+    auto &L = lev[l];
+    std::stable_sort(L.begin(), L.end(), [](const std::pair<double, TrItem> &a, const std::pair<double, TrItem> &b) { return a.first > b.first; });
+    std::vector<double> load(nw, 0.0);
+    std::vector<std::vector<TrItem>> per(nw);
+    for (auto &x : L) {
+      int k = 0;
+      for (int q = 1; q < nw; q++) if (load[q] < load[k]) k = q;
+      load[k] += x.first;
+      per[k].push_back(x.second);
+    }
+    for (int q = 0; q < nw; q++) {
+      P.sched[(size_t) l * nw + q] = (int) P.items.size();
+      P.items.insert(P.items.end(), per[q].begin(), per[q].end());
+    }
+  }
+  P.sched[(size_t) P.nlev * nw] = (int) P.items.size();
+  P.sched_nw = nw; P.sched_vsc = vsc;
 }
 
 static int upload_buf(DevBuf &b, const void *src, size_t n, hipStream_t s)
@@ -282,6 +379,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
     P->st = fresh.st; P->esc = fresh.esc; P->rtmpl = fresh.rtmpl; P->dtmpl = fresh.dtmpl; P->pen = fresh.pen; P->pblk = fresh.pblk;
     P->R = fresh.R; P->RB = fresh.RB; P->RT = fresh.RT; P->root_dx = fresh.root_dx; P->el = fresh.el;
     P->qdbidx = qdbidx; P->semi = semi; P->pty = pty; P->local = fresh.local;
+    P->nlev = fresh.nlev; P->sched_nw = 0; P->sched_vsc = -1;   /* the schedule follows the profile */
     if (upload_buf(P->d_st, P->st.data(), P->st.size() * sizeof(TState), stream) != eslOK ||
         upload_buf(P->d_esc, P->esc.data(), P->esc.size() * 4, stream) != eslOK ||
         upload_buf(P->d_rtmpl, P->rtmpl.data(), P->rtmpl.size() * 4, stream) != eslOK ||
@@ -309,14 +407,26 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
     hw[k].i0 = wi0[w]; hw[k].j0 = wj0[w]; hw[k].dsq_off = off - wi0[w]; hw[k].id = w; hw[k].pad = 0;
   }
   const int W = trsmx->W;
-  const int wpb = infx_wpb_override() ? ESL_MIN(infx_wpb_override(), 4) : 4;
-  const size_t slot_bytes = (((size_t) 6 * P->R + 2 * (size_t) P->RT + 3 * (size_t) (W + 1) * P->RB + 8 * (size_t) (W + 1) + cm->M) * 4 + 255) / 256 * 256;
+  /* wavefronts per window: eight when the windows cannot fill the chip
+   * with four, else four (fewer barriers per row) */
+  int nw = (nwin * 4 < infx_num_cu() * 8) ? 8 : 4;
+  if (infx_wpb_override()) nw = ESL_MIN(infx_wpb_override(), 8);
+  if (P->sched_nw != nw || P->sched_vsc != (want_vsc ? 1 : 0)) {
+    build_tr_schedule(cm, *P, nw, want_vsc ? 1 : 0, fill_L, fill_R, fill_T);
+    if (upload_buf(P->d_sched, P->sched.data(), P->sched.size() * sizeof(int), stream) != eslOK ||
+        upload_buf(P->d_items, P->items.data(), ESL_MAX((size_t) 1, P->items.size()) * sizeof(TrItem), stream) != eslOK)
+      return infx_fail(eslESYS, errbuf, "%s: schedule upload failed", fname);
+  }
+  /* slot: J, L, R rows [2][R] each | T rows [2][RT] | J, L, R deck rings
+   * [RB] each | root history [4][2][W+1] | vsc [M]   (4-byte cells) */
+  const size_t slot_bytes = (((size_t) 6 * P->R + 2 * (size_t) P->RT + 3 * (size_t) P->RB + 8 * (size_t) (W + 1) + cm->M) * 4 + 255) / 256 * 256;
   size_t mem_free = 0, mem_total = 0;
   if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) { (void) hipGetLastError(); mem_free = (size_t) 16 << 30; }
   const size_t budget = ESL_MIN((size_t) 96 << 30, mem_free > ((size_t) 8 << 30) ? mem_free - ((size_t) 8 << 30) : mem_free / 2)
                         + D->d_scratch.cap;
   int nslots = (int) ESL_MIN((int64_t) nwin, (int64_t) (budget / slot_bytes));
-  nslots = ESL_MIN(nslots, infx_num_cu() * 16);
+  const int occ = ESL_MAX(1, infx_tr_occupancy(semi, want_vsc, nw));
+  nslots = ESL_MIN(nslots, infx_num_cu() * occ);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
   size_t wout_bytes = (size_t) nwin * (4 + 4 + 8 + 8) + (want_vsc ? (size_t) nwin * cm->M * 4 : 0) + 1024;
   if (upload_buf(D->d_dsq, packed.data(), packed.size(), stream) != eslOK ||
@@ -338,6 +448,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   A.row_tmpl = P->d_rtmpl.ptr; A.deck_tmpl = P->d_dtmpl.ptr; A.lsum = D->d_lsum.ptr;
   A.pen = (const TrPen *) P->d_pen.ptr; A.npen = (int) P->pen.size();
   A.pen_blk = (const int *) P->d_pblk.ptr; A.pen_nblk = (W + 63) / 64 + 1;
+  A.items = (const TrItem *) P->d_items.ptr; A.sched = (const int *) P->d_sched.ptr; A.nlev = P->nlev; A.nw = nw;
   A.win = (const TrWindow *) D->d_win.ptr; A.nwin = nwin;
   A.scratch = (char *) D->d_scratch.ptr; A.slot_bytes = (int64_t) slot_bytes; A.nslots = nslots;
   A.M = cm->M; A.W = W; A.R = P->R; A.RB = P->RB; A.RT = P->RT; A.root_dx = P->root_dx;
@@ -356,7 +467,7 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   A.win_envj = A.win_envi + nwin;
   A.win_vsc = (float *) (A.win_envj + nwin);
 
-  dim3 block(64 * wpb), grid((nslots + wpb - 1) / wpb);
+  dim3 block(64 * nw), grid(nslots);
   unsigned long long ncand = 0;
   float kms = 0; int launches = 0;
   struct Events {   /* kernel timing; destroyed on every return */
@@ -464,6 +575,29 @@ extern "C" int infx_debug_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx,
   dims[0] = P.R; dims[1] = P.RB; dims[2] = P.RT; dims[3] = (int) P.esc.size(); dims[4] = P.root_dx;
   dims[5] = (int) P.pen.size(); dims[6] = (int) P.pblk.size();
   return (int) P.st.size();
+}
+
+/* The row schedule the truncated kernel runs for nw wavefronts per window
+ * (per-state items if want_vsc): sched[nlev * nw + 1] item offsets and
+ * items[] as (v, b0, b1) triples.  Returns the item count (or -status) and
+ * the level count in *ret_nlev. */
+extern "C" int infx_debug_tr_schedule(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, int nw, int want_vsc,
+                                      int *out_sched, int sched_cap, int *out_items, int items_cap, int *ret_nlev, char *errbuf)
+{
+  int pty, fL, fR, fT;
+  if (!trsmx || qdbidx < 0 || qdbidx > 2 || (semi != TSEMI_CYK && semi != TSEMI_IINSIDE) || nw < 1 || nw > 8) return -eslEINVAL;
+  if ((pty = cm_tr_penalties_IdxForPass(pass_idx)) == -1 || cm->trp == NULL) return -eslEINCOMPAT;
+  if (cm_TrFillFromPassIdx(pass_idx, &fL, &fR, &fT) != eslOK) return -eslEINVAL;
+  TrProfile P;
+  int status = build_tr_profile(cm, trsmx, qdbidx, semi, pty, P, errbuf);
+  if (status != eslOK) return -status;
+  build_tr_schedule(cm, P, nw, want_vsc ? 1 : 0, fL, fR, fT);
+  for (int i = 0; i < (int) P.sched.size() && i < sched_cap; i++) out_sched[i] = P.sched[i];
+  for (int i = 0; i < (int) P.items.size() && 3 * i + 2 < items_cap; i++) {
+    out_items[3 * i] = P.items[i].v; out_items[3 * i + 1] = P.items[i].b0; out_items[3 * i + 2] = P.items[i].b1;
+  }
+  *ret_nlev = P.nlev;
+  return (int) P.items.size();
 }
 
 /* Attach truncated-scanner resources to trsmx without touching a device: a

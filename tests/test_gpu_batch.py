@@ -114,3 +114,36 @@ def test_batch_against_oracle_model_sizes(algo):
         best = max(o["sc"] for o in mine)
         assert np.float32(got[m]["sc"]).view(np.uint32) == np.float32(best).view(np.uint32), (name, got[m]["sc"], best)
     assert sum(len(x["hits"]) for x in got) > 0
+
+
+def test_batch_graph_replays_equal_direct_launches():
+    """The batched launch through a captured graph: three replays of one
+    launch shape, with direct launches and host synchronisations between
+    them, then a different shape (a new capture) and a replay of it; every
+    result bit-identical to the direct launch of the same call."""
+    names = ["tRNA.c.cm", "synth_multiloop350.cm", "Cobalamin.c.cm"]
+    cms = [load_cm(n) for n in names]
+    dsq = _target(20000, seed=63)
+    cutoffs = [-5.0, 0.0, -2.0]
+    wins_a = [(i, j, m) for m, ws in WINDOWS.items() for (i, j) in ws]
+    wins_b = [w for w in wins_a if w[2] != 1]
+    key = lambda h: (h[0], h[1])
+
+    def same(x, y):
+        for m in range(len(cms)):
+            assert hits_equal(sorted(x[m]["hits"], key=key), sorted(y[m]["hits"], key=key)), names[m]
+            assert np.float32(x[m]["sc"]).view(np.uint32) == np.float32(y[m]["sc"]).view(np.uint32), names[m]
+
+    direct_a = ia.ScanBatch("cyk", cms, 1, dsq, wins_a, cutoffs)
+    direct_b = ia.ScanBatch("cyk", cms, 1, dsq, wins_b, cutoffs)
+    try:
+        for rep, wins, want in ((0, wins_a, direct_a), (1, wins_a, direct_a), (2, wins_a, direct_a),
+                                (3, wins_b, direct_b), (4, wins_b, direct_b)):
+            print("case batch graph replay", rep, len(wins), flush=True)
+            ia.set_batch_graph(True)
+            got = ia.ScanBatch("cyk", cms, 1, dsq, wins, cutoffs)
+            ia.set_batch_graph(False)
+            same(got, want)
+            same(ia.ScanBatch("cyk", cms, 1, dsq, wins, cutoffs), want)   # a direct launch between replays
+    finally:
+        ia.set_batch_graph(False)

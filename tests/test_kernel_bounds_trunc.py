@@ -11,8 +11,11 @@ full band bounds every row and window length):
             base[c] + d, in [0, R)
   T rows    cell d of bifurcation c at tbase[c] + d, in [0, RT), inside
             c's own stretch [tbase[c], tbase[c] + dx[c]]
-  decks     BEGL cells at base[c] + d, in [0, RB); the ring row is j mod
-            (W' + 1) with W' <= W, and the slot holds W + 1 deck rows
+  decks     BEGL state c keeps a ring of ring_mask + 1 rows of ring_stride
+            cells; cell d of ring row r at base[c] + r * stride + d, in
+            [0, RB) for every r; a bifurcation reads rows j - k for k up to
+            its BEGR band (kdx_y), so the ring must hold kdx_y + 1 rows, and
+            the rings of different states must not overlap
   emission  [0, nesc) for residue codes 0..15
 
 Every read of a child's cell (c, d') must also fall inside the band the
@@ -40,7 +43,8 @@ TSTATE = np.dtype([("kind", "<i4"), ("emitmode", "<i4"), ("cnum", "<i4"), ("dx",
                    ("tbase", "<i4"), ("esc_off", "<i4"), ("lesc_off", "<i4"), ("resc_off", "<i4"),
                    ("detached", "<i4"), ("first_r", "<i4"), ("init_ok", "<i4"), ("banded", "<i4"),
                    ("kdx_y", "<i4"), ("kdx_w", "<i4"), ("endsc", "<u4"), ("tsc", "<u4", 6),
-                   ("child_base", "<i4", 6), ("pad", "<i4", 4)])
+                   ("child_base", "<i4", 6), ("ring_mask", "<i4"), ("ring_stride", "<i4"), ("level", "<i4"),
+                   ("pad", "<i4")])
 TRPEN = np.dtype([("y", "<i4"), ("base", "<i4"), ("tbase", "<i4"), ("dx", "<i4"), ("is_b", "<i4"), ("pen", "<u4")])
 assert TSTATE.itemsize == 128 and TRPEN.itemsize == 24
 
@@ -84,9 +88,16 @@ class Checker:
         if d.size == 0:
             return
         p = self.p
-        base = int(p["st"][c]["base"])
+        s = p["st"][c]
+        base = int(s["base"])
         size = p["R"] if buf == "row" else p["RB"]
-        idx = base + d
+        if buf == "row":
+            idx = base + d
+        else:                                  # every ring row: first and last
+            if int(s["ring_stride"]) != p["hi"][c] - p["lo"][c] + 1:
+                self._err("deck stride %d of state %d is not its band width" % (int(s["ring_stride"]), c))
+            last = int(s["ring_mask"]) * int(s["ring_stride"])
+            idx = np.concatenate([base + d, base + last + d])
         if idx.min() < 0 or idx.max() >= size:
             self._err("%s index [%d, %d] outside [0, %d) for state %d %s" % (buf, idx.min(), idx.max(), size, c, ctx))
         if d.min() < p["lo"][c] or d.max() > p["hi"][c]:
@@ -129,6 +140,11 @@ def replay(cm, p, states=True):
         if kind == KIND_B:
             w, y = c.cfirst[v], c.cnum[v]
             assert int(s["child_base"][0]) == int(p["st"][w]["base"]) and int(s["child_base"][1]) == int(p["st"][y]["base"])
+            assert int(s["child_base"][2]) == int(p["st"][w]["ring_mask"]) and int(s["child_base"][3]) == int(p["st"][w]["ring_stride"])
+            ring = int(p["st"][w]["ring_mask"]) + 1
+            kdy = int(s["kdx_y"]) if s["banded"] else cm.W
+            if ring < kdy + 1 or ring & (ring - 1):
+                chk._err("deck ring of %d has %d rows for k up to %d %s" % (w, ring, kdy, ctx))
             for d in ds:
                 if s["banded"]:
                     kmin, kmax = max(0, d - int(s["kdx_w"])), min(int(s["kdx_y"]), d)
@@ -147,6 +163,16 @@ def replay(cm, p, states=True):
                     chk.cells("deck", w, d, ctx)               # L: w alone
                 if kmax == d:
                     chk.cells("row", y, d, ctx)                # R: y alone
+            # a block walks k over the union of its lanes' ranges: every lane
+            # loads the BEGR cell of each such k; lanes outside their range
+            # read deck cell 0
+            for d0 in range(1, dx + 1, 64):
+                dl = min(d0 + 63, dx)
+                kb0 = max(0, d0 - int(s["kdx_w"])) if s["banded"] else 0
+                kb1 = min(int(s["kdx_y"]), dl) if s["banded"] else dl
+                if kb0 <= kb1:
+                    chk.cells("row", y, np.arange(kb0, kb1 + 1), ctx + " union")
+                chk.cells("deck", w, 0, ctx + " idle")
             chk.cells("row", v, ds, ctx)
             chk.tcells(v, ds, ctx)
         elif kind == KIND_BEGL:
@@ -206,9 +232,17 @@ def replay(cm, p, states=True):
             ds = np.arange(64 * b + 1, hi + 1)
             if ds.size:
                 chk.cells("row", y, ds, "(root begin %d)" % y)
+                chk.cells("row", y, 1, "(root begin %d, lanes past its band)" % y)
                 if q["is_b"]:
                     assert int(q["tbase"]) == int(p["st"][y]["tbase"])
                     chk.tcells(y, ds, "(root begin %d)" % y)
+    # deck rings of different BEGL states do not overlap
+    rings = sorted((int(p["st"][v]["base"]) + int(p["lo"][v]),
+                    int(p["st"][v]["base"]) + int(p["st"][v]["ring_mask"]) * int(p["st"][v]["ring_stride"]) + int(p["hi"][v]))
+                   for v in range(1, c.M) if p["st"][v]["kind"] == KIND_BEGL and p["lo"][v] <= p["hi"][v])
+    for (a0, a1), (b0, _) in zip(rings, rings[1:]):
+        if a1 >= b0:
+            chk._err("deck rings overlap: [%d, %d] and %d" % (a0, a1, b0))
     # T stretches of different bifurcations do not overlap
     tb = sorted((int(p["st"][v]["tbase"]), int(p["st"][v]["dx"])) for v in range(1, c.M) if p["st"][v]["kind"] == KIND_B)
     for (t0, d0), (t1, _) in zip(tb, tb[1:]):
@@ -252,3 +286,59 @@ def test_trunc_checker_catches_planted_overreach():
              and p["st"][v]["kind"] != KIND_SKIP and p["st"][v]["dx"] >= 1)
     q = dict(p, nesc=int(p["st"][e]["esc_off"]) + 15)
     assert any("emission index" in e for e in replay(cm, q).errors)
+
+
+def tr_schedule(cm, qdbidx, semi, pass_idx, nw, vsc):
+    lib = ia.lib()
+    f = lib.infx_debug_tr_schedule
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                  C.c_int, C.POINTER(C.c_int), C.c_char_p]
+    cap = 4 * cm.M * ((cm.W + 63) // 64 + 1) + 64
+    sched = np.zeros(cap, dtype=np.int32)
+    items = np.zeros(3 * cap, dtype=np.int32)
+    nlev = C.c_int()
+    eb = C.create_string_buffer(128)
+    n = f(C.cast(cm.ptr, C.c_void_p), cm.c.trsmx, qdbidx, semi, pass_idx, nw, int(vsc), sched.ctypes.data, cap,
+          items.ctypes.data, len(items), C.byref(nlev), eb)
+    assert n >= 0, (n, eb.value)
+    L = nlev.value
+    return L, sched[:L * nw + 1], items[:3 * n].reshape(n, 3)
+
+
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Cobalamin.c.cm", "synth_nested1500.cm"])
+def test_trunc_row_schedule(name):
+    """The row schedule the truncated kernel runs: every state with cells is
+    covered, block by block, exactly once; an item sits at its state's level;
+    every child whose same-row cells a state reads (B: BEGL and BEGR) is at a
+    lower level, so the barrier after each level orders every same-row read
+    after its write; IL chains (and every state, when per-state maxima are
+    wanted) are one item on one wavefront."""
+    cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=True, trunc=True, scanmx=True, trscanmx=True)
+    c = cm.c
+    p = tr_profile(cm, 1, 0, ia.PLI_PASS_5P_AND_3P_FORCE)
+    st = p["st"]
+    lev = st["level"]
+    for v in range(1, c.M):
+        if lev[v] < 0:
+            assert st[v]["kind"] == KIND_SKIP or st[v]["dx"] < 1
+            continue
+        kids = [c.cfirst[v], c.cnum[v]] if st[v]["kind"] == KIND_B else [c.cfirst[v] + k for k in range(c.cnum[v])]
+        for ch in kids:
+            if ch != v and lev[ch] >= 0:
+                assert lev[ch] < lev[v], (v, ch, lev[v], lev[ch])
+    for nw, vsc in ((1, False), (4, False), (8, False), (4, True)):
+        L, sched, items = tr_schedule(cm, 1, 0, ia.PLI_PASS_5P_AND_3P_FORCE, nw, vsc)
+        assert L == lev.max() + 1 and sched[0] == 0 and sched[-1] == len(items)
+        assert all(sched[i] <= sched[i + 1] for i in range(len(sched) - 1))
+        cover = {}
+        for l in range(L):
+            for wv in range(nw):
+                for v, b0, b1 in items[sched[l * nw + wv]:sched[l * nw + wv + 1]]:
+                    assert lev[v] == l and 0 <= b0 < b1
+                    if vsc or st[v]["kind"] == KIND_IL:
+                        assert (b0, b1) == (0, (st[v]["dx"] + 63) // 64), (v, b0, b1)
+                    for b in range(b0, b1):
+                        assert (v, b) not in cover
+                        cover[(v, b)] = True
+        want = {(v, b) for v in range(1, c.M) if lev[v] >= 0 for b in range((st[v]["dx"] + 63) // 64)}
+        assert set(cover) == want
