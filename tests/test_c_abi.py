@@ -52,3 +52,18 @@ def test_truncated_matrix_teardown_under_asan():
         r = subprocess.run([exe, "teardown", os.path.join(CMDIR, name)], capture_output=True, text=True, env=env, timeout=300)
         assert r.returncode == 0 and "teardown ok" in r.stdout, r.stdout + r.stderr[-3000:]
         assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+
+
+def test_dlopen_binding_beside_same_named_host_functions():
+    """tests/c/dlopen_check.c defines FastCYKScan, FreeCM, cm_Configure,
+    cm_scan_mx_Create, cm_tophits_* ... itself (as an Infernal binary does) and
+    exports them (-rdynamic); it loads the library with dlopen(RTLD_LOCAL) and
+    calls cm_Configure, FastCYKScan and RefTrCYKScan through dlsym.  Its own
+    definitions abort if called: the library's calls among its own functions
+    must bind inside it (the library is linked -Bsymbolic)."""
+    exe = build("dlopen_check")
+    lib = os.path.join(ROOT, "infernal_amd", "csrc", "libinfernal_gpu.so")
+    r = subprocess.run([exe, lib, os.path.join(CMDIR, "tRNA.c.cm")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "dlopen ok" in r.stdout, r.stdout + r.stderr
+    status = {int(ln.split()[2]) for ln in r.stdout.splitlines() if "status" in ln}
+    assert status in ({0}, {100}), r.stdout

@@ -59,3 +59,30 @@ def test_c_caller_matches_oracle(tmp_path):
             assert g["mode"] == o["mode"], name
         oh = [(h[0], h[1], h[2], h[3], int(np.float32(h[4]).view(np.uint32))) for h in o["hits"]]
         assert g["hits"] == oh, (name, g["hits"], oh)
+
+
+def test_dlopen_binding_matches_oracle():
+    """tests/c/dlopen_check.c: a host that defines the reference's own names
+    (FastCYKScan, FreeCM, cm_Configure, ...) loads the library with
+    dlopen(RTLD_LOCAL) and calls cm_Configure, FastCYKScan and RefTrCYKScan
+    through dlsym on a tRNA; both best scores bit-exact with the oracle."""
+    exe = os.path.join(CDIR, "dlopen_check")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", CDIR, "dlopen_check"], check=True, capture_output=True)
+    lib = os.path.join(ROOT, "infernal_amd", "csrc", "libinfernal_gpu.so")
+    print("case dlopen_check tRNA.c.cm", flush=True)
+    r = subprocess.run([exe, lib, os.path.join(ROOT, "tests", "golden", "cm", "tRNA.c.cm")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "dlopen ok" in r.stdout, r.stdout + r.stderr
+    got = {ln.split()[0]: ln.split() for ln in r.stdout.splitlines() if "status" in ln}
+    seq = "GCCGGGAUAGCUCAGUUGGUAGAGCAGAGGACUGAAAAUCCUCGUGUCACCGGUUCGAUUCCGGUCCCGGCACCA"
+    dsq = ia.digitize(seq)
+    L = len(seq)
+    cm = load_cm("tRNA.c.cm", local=True, trunc=True)
+    o1 = ob.scan(cm, ob.ORACLE_CYK, 1, dsq, 1, L, 0.0)
+    o2 = ob.scan(cm, ob.ORACLE_TRCYK, 1, dsq, 1, L, 0.0, pass_idx=ia.PLI_PASS_5P_AND_3P_FORCE)
+    assert int(got["FastCYKScan"][2]) == 0 and int(got["RefTrCYKScan"][2]) == 0, r.stdout
+    assert int(got["FastCYKScan"][4], 16) == int(np.float32(o1["sc"]).view(np.uint32)), (r.stdout, o1["sc"])
+    assert int(got["RefTrCYKScan"][4], 16) == int(np.float32(o2["sc"]).view(np.uint32)), (r.stdout, o2["sc"])
+    assert int(got["RefTrCYKScan"][6]) == o2["mode"], r.stdout
+    assert o1["sc"] > 20.0
