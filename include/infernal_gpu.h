@@ -467,8 +467,12 @@ int infx_debug_tr_schedule(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi,
  *                                      doubled until the tail is negligible
  *   infx_BandCalculationEngine         replaces BandCalculationEngine
  *                                      (src/cm_qdband.c:163) for one Z;
- *                                      eslERANGE if Z is too small */
-int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W);
+ *                                      eslERANGE if Z is too small.  Its
+ *                                      ret_gamma, as the reference's, returns
+ *                                      the densities gamma_v(0..Z) (here into
+ *                                      a caller buffer of M*(Z+1) doubles,
+ *                                      or NULL) */
+int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W, double *ret_gamma);
 int infx_CalculateQueryDependentBands(CM_t *cm, char *errbuf, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W, int *ret_Z);
 
 /* Many windows wi[w]..wj[w] (sorted, disjoint, 1-based) of one target in one

@@ -60,15 +60,17 @@ static void local_begin_probs(const CM_t *cm, float p_internal_start, std::vecto
   for (int nd = 2; nd < cm->nodes; nd++)
     if (cm->ndtype[nd] == MATP_nd || cm->ndtype[nd] == MATL_nd || cm->ndtype[nd] == MATR_nd || cm->ndtype[nd] == BIF_nd) nstarts++;
   begin.assign(cm->M, 0.f);
-  begin[cm->nodemap[1]] = 1.f - p_internal_start;
+  begin[cm->nodemap[1]] = (float) (1. - (double) p_internal_start);   /* double, then rounded: as the reference */
   const float p = p_internal_start / (float) nstarts;
   for (int nd = 2; nd < cm->nodes; nd++)
     if (cm->ndtype[nd] == MATP_nd || cm->ndtype[nd] == MATL_nd || cm->ndtype[nd] == MATR_nd || cm->ndtype[nd] == BIF_nd)
       begin[cm->nodemap[nd]] = p;
 }
 
-/* src/cm_qdband.c:163.  Returns eslOK, eslERANGE (Z too small), eslEINVAL. */
-extern "C" int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W)
+/* src/cm_qdband.c:163.  Returns eslOK, eslERANGE (Z too small), eslEINVAL.
+ * ret_gamma (optional, caller-allocated [M][Z+1]): each state's density as its
+ * parents used it (after the renormalisation); E states hold the end density. */
+extern "C" int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, double beta_W, int *ret_W, double *ret_gamma)
 {
   if (qdbinfo != NULL && ((qdbinfo->beta2 - qdbinfo->beta1) > 1E-20)) return eslEINVAL;
   const int M = cm->M;
@@ -94,6 +96,7 @@ extern "C" int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, 
     const int st = cm->sttype[v];
     if (st == E_st) {
       if (qdbinfo) { qdbinfo->dmin1[v] = qdbinfo->dmax1[v] = 0; qdbinfo->dmin2[v] = qdbinfo->dmax2[v] = 0; }
+      if (ret_gamma) memcpy(ret_gamma + (size_t) v * (Z + 1), gend.data(), sizeof(double) * (Z + 1));
       continue;
     }
     std::vector<double> &g = gamma[v];
@@ -122,6 +125,7 @@ extern "C" int infx_BandCalculationEngine(CM_t *cm, int Z, CM_QDBINFO *qdbinfo, 
     if (v == 0) pdf = dsum(gamma[0].data(), Z + 1);
     if (pdf <= 0.999 || g[Z] > max_beta * DBL_EPSILON) return eslERANGE;
     if (pdf > 1.0) dnorm(g.data(), Z + 1);
+    if (ret_gamma) memcpy(ret_gamma + (size_t) v * (Z + 1), g.data(), sizeof(double) * (Z + 1));
 
     if (qdbinfo != NULL) {
       pdf = 0.;
@@ -165,7 +169,7 @@ extern "C" int infx_CalculateQueryDependentBands(CM_t *cm, char *errbuf, CM_QDBI
   if (qdbinfo != NULL && ((qdbinfo->beta2 - qdbinfo->beta1) > 1E-20))
     return infx_fail(eslEINVAL, errbuf, "Calculating QDBs, qdbinfo->beta1 < qdbinfo->beta2");
   int Z = cm->clen * 4;
-  while ((status = infx_BandCalculationEngine(cm, Z, qdbinfo, beta_W, ret_W)) != eslOK) {
+  while ((status = infx_BandCalculationEngine(cm, Z, qdbinfo, beta_W, ret_W, NULL)) != eslOK) {
     if (status != eslERANGE) return infx_fail(status, errbuf, "Calculating QDBs, unexpected error");
     Z *= 2;
     if (Z > cm->clen * 1000) return infx_fail(eslEINCONCEIVABLE, errbuf, "Calculating QDBs, Z got insanely large (> 1000*clen)");

@@ -130,6 +130,30 @@ def test_build_reproduces_tutorial_models(name):
 
 
 @need_ref
+@pytest.mark.parametrize("name", ["tRNA5", "5S_rRNA", "Cobalamin"])
+def test_bands_of_rebuilt_models_exact_but_for_ties(name):
+    """The band engine on the models rebuilt from the tutorial alignments,
+    whose transition scores are written to six decimals (cmbuild's own
+    bands came from its unrounded probabilities, the shipped files print
+    three): W and every band column equal the shipped file's, except where
+    the tail mass at the contested length is within 1e-3 of beta (one column
+    of the three models: Cobalamin state 40's dmax2, 7.8e-5 from beta2)."""
+    import infernal_amd as ia
+    from test_qdband import contested, engine_bands, qdb_bands
+    aseqs, gc = cb.read_stockholm(os.path.join(REF, "tutorial", name + ".sto"))
+    text, info = cb.build_cm_from_alignment(cb.digitize_alignment(aseqs), gc["SS_cons"], name, PRIOR)
+    shipped = ia.CM.read(os.path.join(CMDIR, name + ".c.cm"))
+    cm = ia.CM.from_text(text)
+    got, W, Z, gamma = engine_bands(cm, cm.c.beta_W)
+    assert W == shipped.W
+    q = cm.c.qdbinfo.contents
+    ties = contested(got, qdb_bands(shipped), gamma, q.beta1, q.beta2)
+    assert len(ties) <= 1
+    for nm, v, f, o, n, ratio in ties:
+        assert abs(f - o) == 1 and abs(ratio - 1.0) <= 1e-3, (nm, v, f, o, ratio)
+
+
+@need_ref
 def test_pb_weights_and_match_columns():
     """Weights sum to nseq; an all-identical alignment weighs sequences
     equally; a gap-majority column is an insert column."""
