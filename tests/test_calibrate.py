@@ -171,3 +171,47 @@ def test_exp_lines_round_trip(tmp_path):
         a, b = infos[m], back[m]
         assert abs(a.lambda_ - b.lambda_) < 1e-5 and abs(a.mu_extrap - b.mu_extrap) < 1e-5
         assert a.nrandhits == b.nrandhits and abs(a.tailp - b.tailp) < 1e-6
+
+
+def test_read_windows_cover_with_maxw_overlap():
+    """esl_sqio_ReadWindow as cmsearch reads a sequence: windows of `window`
+    new residues, each after the first starting maxW residues inside the
+    previous one; every stretch of at most maxW residues lies inside a
+    window."""
+    from infernal_amd import pipeline as pl
+    for L, maxW, window in ((1, 50, 100), (100, 50, 100), (101, 50, 100), (1000, 90, 100), (250000, 111, 100000)):
+        ws = pl.read_windows(L, maxW, window)
+        assert ws[0] == (1, min(L, window)) and ws[-1][1] == L
+        for (a0, b0), (a1, b1) in zip(ws, ws[1:]):
+            assert a1 == max(1, b0 - maxW + 1) and b1 == min(L, b0 + window)
+        for s in range(1, L + 1, 7):
+            e = min(L, s + maxW - 1)
+            assert any(a <= s and e <= b for a, b in ws), (L, s, e)
+
+
+def test_pipeline_passes_follow_terminus_rules():
+    """The passes cm_Pipeline runs on a window (src/cm_pipeline.c:1375-1412,
+    default truncation): standard always; 5' only with the 5' end, over the
+    first maxW residues; 3' only with the 3' end, over the last maxW; 5' and
+    3' with both ends and a window of at most maxW residues; none truncated
+    with truncation off."""
+    from infernal_amd import pipeline as pl
+    P = ia.PLI_PASS_STD_ANY, ia.PLI_PASS_5P_ONLY_FORCE, ia.PLI_PASS_3P_ONLY_FORCE, ia.PLI_PASS_5P_AND_3P_FORCE
+    assert pl.pipeline_passes(1000, False, False, 100) == [(P[0], 1, 1000)]
+    assert pl.pipeline_passes(1000, True, False, 100) == [(P[0], 1, 1000), (P[1], 1, 100)]
+    assert pl.pipeline_passes(1000, False, True, 100) == [(P[0], 1, 1000), (P[2], 901, 1000)]
+    assert pl.pipeline_passes(1000, True, True, 100) == [(P[0], 1, 1000), (P[1], 1, 100), (P[2], 901, 1000)]
+    assert pl.pipeline_passes(80, True, True, 100) == [(P[0], 1, 80), (P[1], 1, 80), (P[2], 1, 80), (P[3], 1, 80)]
+    assert pl.pipeline_passes(80, True, True, 100, trunc=False) == [(P[0], 1, 80)]
+
+
+def test_search_overlap_removal_best_first_per_strand():
+    """Hits of one strand sharing a residue: the better one stays (equal
+    scores: the lower start); the other strand is independent."""
+    from infernal_amd import pipeline as pl
+    hs = [{"start": 10, "stop": 50, "strand": "+", "score": 5.0}, {"start": 40, "stop": 90, "strand": "+", "score": 7.0},
+          {"start": 91, "stop": 120, "strand": "+", "score": 1.0}, {"start": 60, "stop": 20, "strand": "-", "score": 3.0},
+          {"start": 100, "stop": 55, "strand": "-", "score": 3.0}, {"start": 200, "stop": 150, "strand": "+", "score": 2.0}]
+    kept = pl.remove_overlaps(hs)
+    got = sorted((h["strand"], h["start"], h["stop"]) for h in kept)
+    assert got == [("+", 40, 90), ("+", 91, 120), ("+", 200, 150), ("-", 60, 20)]
