@@ -166,24 +166,38 @@ def cyk_1500(args, ia, synth, threads):
 
 
 def batch_leg(args, ia, synth):
-    """Many models in one launch: every model under tests/golden/cm (local,
-    FastCYKScan QDB1), --batch-windows windows of 1.2 W each per model,
-    sampled from a 2nd-order Markov target with each model's emitted
-    homologs planted.  infx_FastScanBatch (one launch for all models) against
-    one infx_FastScanWindows launch per model: total cells/s, and the
-    per-model latency distribution of the per-model launches (every model
-    of the batched launch finishes with the launch)."""
-    cmdir = os.path.join(ROOT, "tests", "golden", "cm")
-    names = sorted(f for f in os.listdir(cmdir) if f.endswith(".cm"))
+    """Many models in one launch.  Models: every .cm / .cm.gz file of
+    --batch-cmdir (default: the shipped models under tests/golden/cm; the
+    many-models workload uses the 200 synthetic models that
+    tests/golden/make_batch_cms.py builds), local mode, FastCYKScan QDB1.
+    Target: --batch-mb Mb of 2nd-order Markov background with two emitted
+    homologs of every model planted.  Windows: --batch-windows per model, of
+    1.2 W each, at seeded disjoint positions.  Reported: one
+    infx_FastScanBatch launch for all models (direct, then replayed from a
+    captured graph) against one infx_FastScanBatch launch per model (cells/s,
+    per-model latency p50/p99: every model of the batched launch finishes
+    with the launch), hits identical between the two."""
+    import gzip
+    cmdir = args.batch_cmdir or os.path.join(ROOT, "tests", "golden", "cm")
+    names = sorted(f for f in os.listdir(cmdir) if f.endswith(".cm") or f.endswith(".cm.gz"))
+
+    def read(nm):
+        p = os.path.join(cmdir, nm)
+        if nm.endswith(".gz"):
+            with gzip.open(p, "rt") as fh:
+                return ia.CM.from_text(fh.read())
+        return ia.CM.read(p)
+
+    t0 = time.perf_counter()
     cms = []
     for nm in names:
-        cm = ia.CM.read(os.path.join(cmdir, nm))
+        cm = read(nm)
         cm.configure(local=True)
         cms.append(cm)
     rng = np.random.default_rng(404)
     frags = []
     for nm in names:
-        frags += synth.homologs(ia.CM.read(os.path.join(cmdir, nm)), 2, seed=int(rng.integers(1 << 30)))
+        frags += synth.homologs(read(nm), 2, seed=int(rng.integers(1 << 30)))
     n = int(args.batch_mb * 1e6)
     bg = synth.markov2_background(n, gc=0.45, seed=405)
     codes, _ = synth.plant(bg, frags, seed=406)
@@ -195,41 +209,114 @@ def batch_leg(args, ia, synth):
         ws = [(int(a), int(a) + wl - 1) for a in starts]
         per.append(ws)
         wins += [(i, j, m) for (i, j) in ws]
+    log("many models: %d models (W %d-%d), %d windows, set up in %.1f s" % (len(cms), min(c.W for c in cms), max(c.W for c in cms),
+                                                                          len(wins), time.perf_counter() - t0))
     cut = [0.0] * len(cms)
-    ia.ScanBatch("cyk", cms, 1, dsq, wins[:1], cut)                      # first-use set-up
-    t0 = time.perf_counter()
-    got = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
-    tb = time.perf_counter() - t0
-    cells = ia.last_call_timing()["cells"]
-    lat, same = [], True
-    for m, cm in enumerate(cms):
-        ia.ScanWindows("cyk", cm, 1, dsq, per[m][:1], 0.0)
-        t1 = time.perf_counter()
-        r = ia.ScanWindows("cyk", cm, 1, dsq, per[m], 0.0)
-        lat.append(time.perf_counter() - t1)
+    ia.pin_target(dsq)
+    try:
+        t0 = time.perf_counter()
+        ia.ScanBatch("cyk", cms, 1, dsq, [per[m][0] + (m,) for m in range(len(cms))], cut)   # every model's set-up
+        log("many models: first use (profiles, device tables) %.1f s" % (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        got = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
+        tb = time.perf_counter() - t0
+        cells = ia.last_call_timing()["cells"]
+        kb = ia.last_call_timing()["kernel_ms"] / 1e3
+        log("batched: one launch %.2f s (kernel %.2f s), %.3g cells, %.1f Gcells/s" % (tb, kb, cells, cells / tb / 1e9))
+        tg = []
+        ia.set_batch_graph(True)
+        try:
+            for rep in range(2):              # capture, then replay
+                t0 = time.perf_counter()
+                gg = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
+                tg.append(time.perf_counter() - t0)
+                log("batched: %s %.2f s" % ("graph capture and run" if rep == 0 else "graph replay", tg[-1]))
+        finally:
+            ia.set_batch_graph(False)
         key = lambda h: (h[0], h[1])
-        same = same and sorted(r["hits"], key=key) == sorted(got[m]["hits"], key=key) and \
-            np.float32(r["sc"]).view(np.uint32) == np.float32(got[m]["sc"]).view(np.uint32)
-    log("batched: %d models, %d windows: one launch %.2f s, one launch per model %.2f s" % (len(cms), len(wins), tb, sum(lat)))
-    tuned = None
-    if args.batch_tune:
-        from infernal_amd import autotune
-        best, trials = autotune.tune("cyk", cms, 1, dsq, wins, cut, sample=0.1)
-        with autotune.apply(best, max(cm.W for cm in cms)):
-            t0 = time.perf_counter()
-            got2 = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
-            tt = time.perf_counter() - t0
-        same = same and all(sorted(a["hits"]) == sorted(b["hits"]) for a, b in zip(got, got2))
-        tuned = {"config": best, "wall_s": round(tt, 3), "gcells_per_s": round(cells / tt / 1e9, 3),
-                 "trials": [[c, round(t, 4)] for c, t in trials]}
-        log("batched, tuned %s: %.2f s" % (best, tt))
-    return {"models": ["%s (W %d)" % (nm[:-3], cm.W) for nm, cm in zip(names, cms)], "windows": len(wins),
-            "windows_per_model": args.batch_windows, "window_len": "1.2 W", "target_mb": args.batch_mb, "cells": cells,
-            "batched": {"wall_s": round(tb, 3), "gcells_per_s": round(cells / tb / 1e9, 3)},
+        same = all(sorted(a["hits"], key=key) == sorted(b["hits"], key=key) for a, b in zip(got, gg))
+        tuned = None
+        if args.batch_tune:
+            from infernal_amd import autotune
+            best, trials = autotune.tune("cyk", cms, 1, dsq, wins, cut, sample=0.1)
+            with autotune.apply(best, max(cm.W for cm in cms)):
+                t0 = time.perf_counter()
+                gt = ia.ScanBatch("cyk", cms, 1, dsq, wins, cut)
+                tt = time.perf_counter() - t0
+            same = same and all(sorted(a["hits"], key=key) == sorted(b["hits"], key=key) for a, b in zip(got, gt))
+            tuned = {"config": best, "wall_s": round(tt, 3), "gcells_per_s": round(cells / tt / 1e9, 3),
+                     "trials": [[c, round(t, 4)] for c, t in trials]}
+            log("many models, tuned %s: %.2f s" % (best, tt))
+        lat = []
+        for m, cm in enumerate(cms):
+            ia.ScanBatch("cyk", [cm], 1, dsq, [(i, j, 0) for (i, j) in per[m][:1]], [0.0])
+            t1 = time.perf_counter()
+            r = ia.ScanBatch("cyk", [cm], 1, dsq, [(i, j, 0) for (i, j) in per[m]], [0.0])[0]
+            lat.append(time.perf_counter() - t1)
+            if (m + 1) % 25 == 0:
+                log("per-model launches: %d of %d models, %.1f s" % (m + 1, len(cms), sum(lat)))
+            same = same and sorted(r["hits"], key=key) == sorted(got[m]["hits"], key=key) and \
+                np.float32(r["sc"]).view(np.uint32) == np.float32(got[m]["sc"]).view(np.uint32)
+    finally:
+        ia.unpin_target()
+    log("many models: one launch %.2f s, graph replay %.2f s, one launch per model %.2f s" % (tb, tg[-1], sum(lat)))
+    return {"models": len(cms), "model_dir": os.path.relpath(cmdir, ROOT) if cmdir.startswith(ROOT) else cmdir,
+            "W_range": [min(c.W for c in cms), max(c.W for c in cms)],
+            "clen_range": [min(c.clen for c in cms), max(c.clen for c in cms)],
+            "windows": len(wins), "windows_per_model": args.batch_windows, "window_len": "1.2 W", "target_mb": args.batch_mb,
+            "cells": cells,
+            "batched": {"wall_s": round(tb, 3), "kernel_s": round(kb, 3), "gcells_per_s": round(cells / tb / 1e9, 3),
+                        "latency_s": round(tb, 3)},
+            "batched_graph_replay": {"wall_s": round(tg[-1], 3), "gcells_per_s": round(cells / tg[-1] / 1e9, 3)},
             "per_model_launches": {"wall_s": round(sum(lat), 3), "gcells_per_s": round(cells / sum(lat) / 1e9, 3),
-                                   "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
-                                   "latency_p99_s": round(float(np.percentile(lat, 99)), 3)},
-            "batched_latency_s": round(tb, 3), "tuned": tuned, "hits_identical": bool(same)}
+                                   "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+                                   "latency_p99_s": round(float(np.percentile(lat, 99)), 4)},
+            "tuned": tuned, "hits_identical": bool(same)}
+
+
+def latency_leg(args, ia, synth):
+    """Single-call latency of the drop-in on short targets: one FastCYKScan
+    (QDB1) and one FastFInsideScan (QDB2) call of the 350-nt model on targets
+    of 2,300 and 8,000 nt (2nd-order Markov background, one planted homolog),
+    wall time of the call (upload, launch, hit download included), median of
+    --latency-reps calls; with the default chunking and with chunks of 2W
+    (the rule for targets that fill the chip), hits compared between the
+    two."""
+    cm = ia.CM.read(args.cm)
+    cm.configure(local=True)
+    W = cm.W
+    out = {"cm": "%s (clen %d, W %d)" % (cm.c.name.decode(), cm.clen, W), "reps": args.latency_reps, "targets": []}
+    key = lambda h: (h[0], h[1])
+    for L in (2300, 8000):
+        bg = synth.markov2_background(L, gc=0.40, seed=900 + L)
+        frags = synth.homologs(ia.CM.read(args.cm), 1, seed=901 + L)
+        codes, _ = synth.plant(bg, frags, seed=902 + L)
+        dsq = ia.dsq_from_codes(codes)
+        rec = {"nt": int(len(codes))}
+        for algo, f, q in (("cyk", ia.FastCYKScan, 1), ("finside", ia.FastFInsideScan, 2)):
+            res = {}
+            for mode, chunk in (("default", 0), ("chunks_2W", 2 * W)):
+                ia.set_launch_params(chunk, 0)
+                try:
+                    f(cm, q, dsq, 1, len(codes), 0.0)                 # first use
+                    ts, hits = [], None
+                    for _ in range(args.latency_reps):
+                        t0 = time.perf_counter()
+                        r = f(cm, q, dsq, 1, len(codes), 0.0)
+                        ts.append(time.perf_counter() - t0)
+                        hits = sorted(r["hits"], key=key)
+                finally:
+                    ia.set_launch_params(0, 0)
+                res[mode] = {"wall_ms": round(1e3 * float(np.median(ts)), 2), "kernel_ms": round(ia.last_call_timing()["kernel_ms"], 2),
+                             "hits": hits}
+            same = res["default"]["hits"] == res["chunks_2W"]["hits"]
+            rec[algo] = {m: {k: v for k, v in r.items() if k != "hits"} for m, r in res.items()}
+            rec[algo]["hits"] = len(res["default"]["hits"])
+            rec[algo]["hits_identical"] = bool(same)
+            rec[algo]["speedup"] = round(res["chunks_2W"]["wall_ms"] / max(res["default"]["wall_ms"], 1e-9), 2)
+            log("latency %s %d nt: %.1f ms (2W chunks %.1f ms)" % (algo, len(codes), res["default"]["wall_ms"], res["chunks_2W"]["wall_ms"]))
+        out["targets"].append(rec)
+    return out
 
 
 def trunc_leg(args, ia, synth, threads):
@@ -453,9 +540,12 @@ def main():
     ap.add_argument("--trunc-only", action="store_true", help="run only the truncated-scan leg and print its JSON")
     ap.add_argument("--batch-windows", type=int, default=0,
                     help="windows per model of the many-models leg (one launch for all models vs one per model); 0 = skip")
-    ap.add_argument("--batch-mb", type=float, default=10.0, help="target the many-models leg samples its windows from")
-    ap.add_argument("--batch-only", action="store_true", help="run only the many-models leg and print its JSON")
+    ap.add_argument("--batch-mb", type=float, default=50.0, help="target the many-models leg samples its windows from")
+    ap.add_argument("--batch-cmdir", default="", help="models of the many-models leg (.cm or .cm.gz; default tests/golden/cm)")
     ap.add_argument("--batch-tune", action="store_true", help="many-models leg: also run with autotuned launch parameters")
+    ap.add_argument("--batch-only", action="store_true", help="run only the many-models leg and print its JSON")
+    ap.add_argument("--latency-only", action="store_true", help="run only the single-call latency leg and print its JSON")
+    ap.add_argument("--latency-reps", type=int, default=5, help="calls per latency case (median reported)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
@@ -490,6 +580,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(batch_leg(args, ia, synth)))
+        return
+    if args.latency_only:
+        print(json.dumps(latency_leg(args, ia, synth)))
         return
 
     cm = ia.CM.read(args.cm)

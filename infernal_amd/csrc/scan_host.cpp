@@ -715,12 +715,24 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
     return infx_fail(eslEINVAL, errbuf, "%s: no workgroup shape fits (W = %d, LDS limit %zu bytes)", fname, W, (size_t) g_lds_max);
   const int64_t resident = (int64_t) occ * wpb * g_ncu;
 
-  /* work split: one chunk per resident wavefront, each >= 2W long so the
-   * W-1 warm-up stays small */
+  /* work split.  A chunk's wavefront sweeps chunk + W - 1 rows (its W - 1
+   * warm-up rows rebuild the cells its first rows read), so a call takes
+   * about that many row times once every chunk has a wavefront.  Targets that
+   * fill the chip with chunks of 2W or more: one chunk per resident
+   * wavefront (the warm-up at most a third of the work).  Shorter targets:
+   * shorter chunks (at least 64 rows, at most 2W), up to two wavefronts per
+   * SIMD, which trades extra warm-up work on an idle chip for latency: a
+   * 2,300-nt target and W = 367 sweeps at most 430 rows per wavefront
+   * instead of 1,100. */
   int64_t chunk = infx_chunk_override();
   if (chunk <= 0) {
-    chunk = (L + resident - 1) / resident;
-    chunk = ESL_MAX(chunk, (int64_t) 2 * W);
+    const int64_t fill = (L + resident - 1) / resident;
+    if (fill >= (int64_t) 2 * W) {
+      chunk = fill;
+    } else {
+      const int64_t conc = ESL_MIN(resident, (int64_t) 8 * g_ncu);
+      chunk = ESL_MIN((int64_t) 2 * W, ESL_MAX((L + conc - 1) / conc, (int64_t) 64));
+    }
     chunk = ESL_MAX(chunk, (int64_t) 64);
   }
   int64_t nchunks64 = 0;
@@ -904,10 +916,10 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   }
 
   double cells = 0;
-  for (int w = 0; w < nwin; w++) {
-    double cw = 0;
-    infx_count_scan_cells(cm, smx, qdbidx, 0, wi[w], wj[w], &cw);
-    cells += cw;
+  {
+    CellRows cr;
+    infx_cell_rows(cm, smx, qdbidx, 0, &cr);
+    for (int w = 0; w < nwin; w++) cells += cr.cells(wj[w] - wi[w] + 1);
   }
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (getenv("INFX_VERBOSE"))
@@ -940,26 +952,31 @@ extern "C" int infx_debug_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int
 
 /* Exact number of (v, j, d) cells the reference computes over j = i0..j0
  * (E states excluded, v = 0 included): bands clipped by jp for jp < W. */
-extern "C" int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells)
+void infx_cell_rows(const CM_t *cm, const void *mx, int qdbidx, int truncated, CellRows *out)
 {
-  int M, Wm; int ***dnA, ***dxA;
-  if (truncated) { const CM_TR_SCAN_MX *t = (const CM_TR_SCAN_MX *) mx; M = t->M; Wm = t->W; dnA = t->dnAAA; dxA = t->dxAAA; }
-  else           { const CM_SCAN_MX *s = (const CM_SCAN_MX *) mx;    M = s->M; Wm = s->W; dnA = s->dnAAA; dxA = s->dxAAA; }
-  const int64_t L = j0 - i0 + 1;
-  const int W = (Wm > L) ? (int) L : Wm;
-  double cells = 0;
-  /* rows 1..W-1 once each, row W for every j with jp >= W; E states are skipped by the scanners */
+  int M, W; int ***dnA, ***dxA;
+  if (truncated) { const CM_TR_SCAN_MX *t = (const CM_TR_SCAN_MX *) mx; M = t->M; W = t->W; dnA = t->dnAAA; dxA = t->dxAAA; }
+  else           { const CM_SCAN_MX *s = (const CM_SCAN_MX *) mx;    M = s->M; W = s->W; dnA = s->dnAAA; dxA = s->dxAAA; }
+  out->W = W;
+  out->row.assign((size_t) W + 1, 0.0);
+  out->pre.assign((size_t) W + 1, 0.0);
   for (int jp = 1; jp <= W; jp++) {
-    double mult = (jp < W) ? 1.0 : (double) (L - W + 1);
     double row = 0;
     for (int v = 0; v < M; v++) {
       if (cm->sttype[v] == E_st) continue;
-      int dn = dnA[qdbidx][jp][v], dx = dxA[qdbidx][jp][v];
+      const int dn = dnA[qdbidx][jp][v], dx = dxA[qdbidx][jp][v];
       if (dx >= dn) row += dx - dn + 1;
     }
-    cells += mult * row;
+    out->row[jp] = row;
+    out->pre[jp] = out->pre[jp - 1] + row;
   }
-  *ret_cells = cells;
+}
+
+extern "C" int infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells)
+{
+  CellRows cr;
+  infx_cell_rows(cm, mx, qdbidx, truncated, &cr);
+  *ret_cells = cr.cells(j0 - i0 + 1);
   return eslOK;
 }
 
@@ -1143,7 +1160,8 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
   struct Ch { int64_t a, b, lo; int m; double cost; };
   std::vector<Ch> ch;
   std::vector<double> cpr(nmodel, 0.0);   /* cells per residue at full band */
-  for (int m = 0; m < nmodel; m++) if (prof[m]) { double c = 0; infx_count_scan_cells(cms[m], smxs[m], qdbidx, 0, 1, Wm[m], &c); cpr[m] = c / Wm[m]; }
+  std::vector<CellRows> crows(nmodel);
+  for (int m = 0; m < nmodel; m++) if (prof[m]) { infx_cell_rows(cms[m], smxs[m], qdbidx, 0, &crows[m]); cpr[m] = crows[m].cells(Wm[m]) / Wm[m]; }
   int64_t base_chunk = infx_chunk_override() > 0 ? infx_chunk_override() : (Ltot + resident - 1) / resident;
   for (int w = 0; w < nwin; w++) {
     const int m = wm[w];
@@ -1307,11 +1325,7 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     }
   }
   double cells = 0;
-  for (int w = 0; w < nwin; w++) {
-    double cw = 0;
-    infx_count_scan_cells(cms[wm[w]], smxs[wm[w]], qdbidx, 0, wi[w], wj[w], &cw);
-    cells += cw;
-  }
+  for (int w = 0; w < nwin; w++) cells += crows[wm[w]].cells(wj[w] - wi[w] + 1);
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (getenv("INFX_VERBOSE"))
     fprintf(stderr, "[infx] %s batch: %d models, %d windows, %d chunks, %d slots x %zu B, wpb %d blocks/CU %d: wall %.1f ms, kernel %.1f ms, %llu candidates\n",

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <vector>
 
 struct DevBuf {
   void  *ptr = nullptr;
@@ -33,6 +34,21 @@ extern "C" int  infx_dev_init(char *errbuf);
 extern "C" void infx_dev_release(void *dev);
 extern "C" void infx_trdev_release(void *trdev);
 extern "C" int  infx_count_scan_cells(const CM_t *cm, const void *mx, int qdbidx, int truncated, int64_t i0, int64_t j0, double *ret_cells);
+/* banded cells of a scan matrix's rows at chunk positions jp = 1..W (E states
+ * skipped): row[jp], and pre[k] = row[1] + ... + row[k].  A window of L
+ * residues scans rows 1..W'-1 once and row W' = min(W, L) for its other
+ * L - W' + 1 positions: cells(L) = pre[W'-1] + (L - W' + 1) row[W'].  Built
+ * once per model and call, O(1) per window. */
+struct CellRows {
+  std::vector<double> row, pre;
+  int W = 0;
+  double cells(int64_t L) const {
+    if (L <= 0 || W <= 0) return 0.0;
+    const int w = (int) (L < W ? L : W);
+    return pre[w - 1] + (double) (L - w + 1) * row[w];
+  }
+};
+void infx_cell_rows(const CM_t *cm, const void *mx, int qdbidx, int truncated, CellRows *out);
 
 hipStream_t infx_stream(void);
 const uint8_t *infx_pinned(const uint8_t *dsq, int64_t x0, int64_t x1);

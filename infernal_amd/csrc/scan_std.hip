@@ -142,10 +142,12 @@ template <> struct SRB<SEMI_FINSIDE> {
    * index 23000, whose entry is 0 (max + 0 == max).  A difference below 23
    * gives (int)(diff*1000) <= 22999 (the float product of the largest float
    * below 23 and 1000 rounds below 23000).  -inf - -inf is NaN, and
-   * fminf(NaN, 23000) is 23000. */
+   * fminf(NaN, 23000) is 23000.  max - min is |a - b| bit for bit (IEEE
+   * subtraction rounds symmetrically; -inf - x is -inf), so the min is not
+   * formed: the abs is a source modifier of the multiply. */
   static __device__ __forceinline__ T op(T a, T b, const T *tbl) {
-    const float mx = fmaxf(a, b), mn = fminf(a, b);
-    const float diff = __fsub_rn(mx, mn);
+    const float mx = fmaxf(a, b);
+    const float diff = fabsf(__fsub_rn(a, b));
     const int   idx  = (int) fminf(__fmul_rn(diff, 1000.0f), 23000.0f);
     return __fadd_rn(mx, tbl[idx]);
   }

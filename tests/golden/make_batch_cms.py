@@ -34,10 +34,20 @@ def plan(n, seed):
 
 def build(job):
     i, clen, nstems, s, prior, outdir = job
-    rng = np.random.default_rng(s)
-    ss = cb.nested_structure(clen, nstems, rng, bp_range=(3, 10))
-    aln = cb.synthetic_alignment(ss, 20, 0.20, rng)
+    sys.setrecursionlimit(100000)      # guide trees of deeply nested stems recurse per node
     name = "batch%03d_%d" % (i, clen)
+    if os.path.exists(os.path.join(outdir, name + ".cm.gz")):
+        return name, clen, 0, 0, nstems
+    rng = np.random.default_rng(s)
+    while True:                          # fewer stems until the structure fits the length
+        try:
+            ss = cb.nested_structure(clen, nstems, rng, bp_range=(3, 10))
+            break
+        except ValueError:
+            if nstems == 1:
+                raise
+            nstems -= 1
+    aln = cb.synthetic_alignment(ss, 20, 0.20, rng)
     text, info = cb.build_cm(ss, aln, name, prior)
     with gzip.open(os.path.join(outdir, name + ".cm.gz"), "wt") as fh:
         fh.write(text)
