@@ -145,9 +145,11 @@ struct TrItem {
   int32_t v, b0, b1;
 };
 
-/* a state with a finite truncated-begin penalty, in ascending order */
+/* a state with a finite truncated-begin penalty, in ascending order.
+ * flags: bit 0 a B state, bit 1 its node's first consensus position is 1,
+ * bit 2 its last is clen (cm_hit_AllowTruncation's lpos / rpos tests) */
 struct TrPen {
-  int32_t y, base, tbase, dx, is_b;
+  int32_t y, base, tbase, dx, flags;
   uint32_t pen;
 };
 
@@ -188,7 +190,11 @@ struct TrScanArgs {
   int            root_dx;
   int            fill_L, fill_R, fill_T;
   uint32_t       el_selfsc;
-  int            cand_mode;   /* 0: sc >= cutoff, 2: sc > 0 (gamma) */
+  int            cand_mode;   /* 0: sc >= cutoff; 1: greedy without NULL3: sc >= cutoff, admitted by the
+                                 pass (cm_hit_AllowTruncation) and above every admitted d < this one
+                                 of its 64-cell block; 2: sc > 0 (gamma) */
+  int            pass_any;    /* the pass admits every hit */
+  int            local;       /* local begins on */
   float          cutoff, env_cutoff;
   TrCand        *cand;
   unsigned long long *ncand;

@@ -59,7 +59,7 @@ __device__ __forceinline__ TrPen load_pen(const TrPen *base, int i)
 {
   CTrPen *q = (CTrPen *) base + i;
   TrPen p;
-  p.y = q->y; p.base = q->base; p.tbase = q->tbase; p.dx = q->dx; p.is_b = q->is_b; p.pen = q->pen;
+  p.y = q->y; p.base = q->base; p.tbase = q->tbase; p.dx = q->dx; p.flags = q->flags; p.pen = q->pen;
   return p;
 }
 /* lane i <- x of lane i-1, lane 0 <- lane 0 of `first` (DPP wave_shr:1) */
@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(64 * TR_MAX_WAVES) tr_scan_kernel(TrScanArgs a
       for (int blk = wid; 64 * blk + 1 <= rdx; blk += nw) {
         const int d = 64 * blk + 1 + lane;
         const bool act = d <= rdx;
-        float bsc = IMPF; int bmode = TRMODE_UNKNOWN_K, br = 0;
+        float bsc = IMPF; int bmode = TRMODE_UNKNOWN_K, br = 0, bfl = 0;
         if (act) {
           Cc hj = HJ[d], hl = HL[d], hr = HR[d], ht = HT[d];
           /* the begins whose band reaches this block, ascending */
@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(64 * TR_MAX_WAVES) tr_scan_kernel(TrScanArgs a
               cj[u] = Jc[pg[u].base + dq];
               cl[u] = fL ? Lc[pg[u].base + dq] : R::imp();
               cr[u] = fR ? Rc[pg[u].base + dq] : R::imp();
-              ct[u] = (fT && pg[u].is_b) ? Tc[pg[u].tbase + dq] : R::imp();
+              ct[u] = (fT && (pg[u].flags & 1)) ? Tc[pg[u].tbase + dq] : R::imp();
             }
 #pragma unroll
             for (int u = 0; u < TR_RU; u++) { hold_t(cj[u]); hold_t(cl[u]); hold_t(cr[u]); hold_t(ct[u]); }
@@ -576,17 +576,47 @@ __global__ void __launch_bounds__(64 * TR_MAX_WAVES) tr_scan_kernel(TrScanArgs a
               if (d > min(rdx, min(min(jp, We), p.dx))) continue;
               const Cc pen = R::bits(p.pen);
               Cc sc2 = R::add(cj[u], pen);
-              if (sc2 > hj) { hj = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_J_K; br = p.y; } }
-              if (fL) { sc2 = R::add(cl[u], pen); if (sc2 > hl) { hl = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_L_K; br = p.y; } } }
-              if (fR) { sc2 = R::add(cr[u], pen); if (sc2 > hr) { hr = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_R_K; br = p.y; } } }
-              if (fT && p.is_b) { sc2 = R::add(ct[u], pen); if (sc2 > ht) { ht = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_T_K; br = p.y; } } }
+              if (sc2 > hj) { hj = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_J_K; br = p.y; bfl = p.flags; } }
+              if (fL) { sc2 = R::add(cl[u], pen); if (sc2 > hl) { hl = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_L_K; br = p.y; bfl = p.flags; } } }
+              if (fR) { sc2 = R::add(cr[u], pen); if (sc2 > hr) { hr = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_R_K; br = p.y; bfl = p.flags; } } }
+              if (fT && (p.flags & 1)) { sc2 = R::add(ct[u], pen); if (sc2 > ht) { ht = sc2; const float f = R::score(sc2); if (f > bsc) { bsc = f; bmode = TRMODE_T_K; br = p.y; bfl = p.flags; } } }
             }
           }
           HJ[d] = hj; HL[d] = hl; HR[d] = hr; HT[d] = ht;
           if (bsc > best) { best = bsc; best_mode = bmode; best_j = j; best_d = d; }   /* a lane's first maximum in scan order */
           if (bsc >= a.env_cutoff) { envi = min(envi, j - d + 1); envj = max(envj, j); }
         }
-        const bool rep = act && ((a.cand_mode == 2) ? (bsc > 0.f && bsc > NOTIMP_F) : (bsc >= a.cutoff && bsc > NOTIMP_F));
+        bool rep;
+        if (a.cand_mode == 2) {
+          rep = act && bsc > 0.f && bsc > NOTIMP_F;
+        } else {
+          const bool q = act && bsc >= a.cutoff && bsc > NOTIMP_F;
+          if (a.cand_mode == 1) {
+            /* src/cm_mx.c:7547 ReportHitsGreedily reports d at this j only if
+             * its score beats every admitted d' < d (cm_hit_AllowTruncation,
+             * src/cm_tophits.c:3048), so a d that an admitted shorter d of
+             * its block beats can never be reported; the host replay
+             * applies the rule across blocks */
+            const int64_t st = j - d + 1;
+            bool adm = a.pass_any || (st == i0 && j == j0);
+            if (!adm) {
+              if (a.local) adm = bmode == TRMODE_J_K || (bmode == TRMODE_L_K && j == j0) || (bmode == TRMODE_R_K && st == i0);
+              else adm = (bmode == TRMODE_J_K && (bfl & 6) == 6) || (bmode == TRMODE_L_K && j == j0 && (bfl & 2)) ||
+                         (bmode == TRMODE_R_K && st == i0 && (bfl & 4));
+            }
+            const bool qa = q && adm;
+            float inc = qa ? bsc : -INFINITY;
+            for (int o = 1; o < 64; o <<= 1) {
+              const float u = __shfl_up(inc, o, 64);
+              if (lane >= o) inc = fmaxf(inc, u);
+            }
+            float ex = __shfl_up(inc, 1, 64);
+            if (lane == 0) ex = -INFINITY;
+            rep = qa && bsc > ex;
+          } else {
+            rep = q;
+          }
+        }
         const unsigned long long m = __ballot(rep);
         if (m) {
           const int n = __popcll(m);

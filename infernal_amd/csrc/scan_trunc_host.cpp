@@ -241,7 +241,15 @@ static int build_tr_profile(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi
     }
     if (!ok) continue;
     if (cm->stid[y] == BEGL_S || cm->sttype[y] == E_st) return infx_fail(eslEINVAL, errbuf, "truncated begin into state %d", y);
-    p.y = y; p.base = base[y]; p.tbase = tbase[y]; p.dx = dx[y]; p.is_b = (cm->sttype[y] == B_st);
+    p.y = y; p.base = base[y]; p.tbase = tbase[y]; p.dx = dx[y];
+    if (!cm->emap) return infx_fail(eslEINVAL, errbuf, "truncated scan needs the model's emit map");
+    {
+      /* src/cm_tophits.c:3048: the consensus span of the begin state's node */
+      const int nd = cm->ndidx[y];
+      const int lpos = (cm->ndtype[nd] == MATP_nd || cm->ndtype[nd] == MATL_nd) ? cm->emap->lpos[nd] : cm->emap->lpos[nd] + 1;
+      const int rpos = (cm->ndtype[nd] == MATP_nd || cm->ndtype[nd] == MATR_nd) ? cm->emap->rpos[nd] : cm->emap->rpos[nd] - 1;
+      p.flags = (cm->sttype[y] == B_st ? 1 : 0) | (lpos == 1 ? 2 : 0) | (rpos == cm->clen ? 4 : 0);
+    }
     P.pen.push_back(p);
   }
   P.root_dx = dx[0];
@@ -455,7 +463,13 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   A.fill_L = fill_L; A.fill_R = fill_R; A.fill_T = fill_T;
   A.el_selfsc = P->el;
   const int greedy = !(cm->search_opts & CM_SEARCH_CMNOTGREEDY);
-  A.cand_mode = !want_hits ? 0 : (greedy ? 0 : 2);
+  /* greedy without NULL3: the kernel drops the candidates the greedy
+   * reporter can never report (a shorter admitted d of the same 64-cell
+   * block scores at least as high); NULL3 changes the scores the reporter
+   * compares, so every candidate goes to the host */
+  A.cand_mode = !want_hits ? 0 : (!greedy ? 2 : (do_null3 ? 0 : 1));
+  A.pass_any = (pass_idx == PLI_PASS_STD_ANY || pass_idx == PLI_PASS_5P_AND_3P_ANY || pass_idx == PLI_PASS_HMM_ONLY_ANY) ? 1 : 0;
+  A.local = (cm->flags & CMH_LOCAL_BEGIN) ? 1 : 0;
   A.cutoff = want_hits ? cutoff : INFINITY;
   A.env_cutoff = env_cutoff;
   A.cand = (TrCand *) D->d_cand.ptr; A.ncand = (unsigned long long *) D->d_ncand.ptr; A.cand_cap = cand_cap;
@@ -511,7 +525,11 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
   outs.resize(nwin);
   for (int w = 0; w < nwin; w++) { outs[w].best = wb[w]; outs[w].mode = wm[w]; outs[w].envi = wi[w]; outs[w].envj = wj[w]; }
   double cells = 0;
-  for (int w = 0; w < nwin; w++) { double c = 0; infx_count_scan_cells(cm, trsmx, qdbidx, 1, wi0[w], wj0[w], &c); cells += c; }
+  {
+    CellRows cr;
+    infx_cell_rows(cm, trsmx, qdbidx, 1, &cr);
+    for (int w = 0; w < nwin; w++) cells += cr.cells(wj0[w] - wi0[w] + 1);
+  }
   infx_timing_add(0, kms, (int64_t) cells, launches);
   return eslOK;
 }

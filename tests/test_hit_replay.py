@@ -91,3 +91,85 @@ def test_greedy_replay_matches_reference_restatement(seed):
     lib.cm_tophits_Destroy(th)
     want = reference(uniq, cutoff)
     assert got == want
+
+
+def _allow(cm, pass_any, local, start, stop, i0, j0, mode, b):
+    """cm_hit_AllowTruncation (src/cm_tophits.c:3048) from the begin state's
+    node span, as the truncated kernel's filter evaluates it."""
+    c = cm.c
+    nd = c.ndidx[b]
+    t = ord(c.ndtype[nd])
+    MATP, MATL, MATR = 1, 2, 3     # node types (src/infernal.h)
+    lpos = c.emap.contents.lpos[nd] if t in (MATP, MATL) else c.emap.contents.lpos[nd] + 1
+    rpos = c.emap.contents.rpos[nd] if t in (MATP, MATR) else c.emap.contents.rpos[nd] - 1
+    if pass_any or (start == i0 and stop == j0):
+        return True
+    if local:
+        return mode == 3 or (mode == 2 and stop == j0) or (mode == 1 and start == i0)
+    return (mode == 3 and lpos == 1 and rpos == c.clen) or (mode == 2 and stop == j0 and lpos == 1) or \
+        (mode == 1 and start == i0 and rpos == c.clen)
+
+
+@pytest.mark.parametrize("local", [True, False])
+@pytest.mark.parametrize("pass_idx", [2, 3, 4])
+def test_truncated_block_record_filter_keeps_every_reported_hit(local, pass_idx):
+    """The truncated kernel, in greedy mode without NULL3, hands the host only
+    the cells (j, d) that are admitted (cm_hit_AllowTruncation for their best
+    mode and begin state), reach the cutoff and beat every admitted cell of
+    a smaller d in the same 64-cell block of d.  ReportHitsGreedily
+    (src/cm_mx.c:7547) reports a cell only when it beats every admitted
+    smaller d of the row, so the dropped cells are never reported: the
+    library's replay gives the same hits from the filtered set as from all
+    cells.  Random rows with coarse scores (ties), all four modes and random
+    begin states, on a window whose first and last rows are in the set."""
+    lib = ia.lib()
+    f = lib.infx_resolve_hits
+    f.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                  C.c_int, C.c_float, C.c_int, C.c_int, C.c_void_p]
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=local, trunc=True, trscanmx=True)
+    rng = np.random.default_rng(100 * pass_idx + local)
+    i0, j0, W = 1001, 1400, 150
+    pass_any = pass_idx in (1, 5, 6)
+    cells = []
+    for j in range(i0, j0 + 1):
+        dmax = min(W, j - i0 + 1)
+        for d in range(1, dmax + 1):
+            if rng.random() < 0.6:
+                cells.append((j, d, float(rng.integers(-10, 30) / 2.0), int(rng.integers(1, cm.M - 1)), int(rng.integers(0, 4))))
+    cells = [c for c in cells if ord(cm.c.sttype[c[3]]) != ia.E_st]
+    kept = []
+    by_row = {}
+    for c in cells:
+        by_row.setdefault(c[0], []).append(c)
+    cutoff = 0.0
+    for j, row in by_row.items():
+        row.sort(key=lambda c: c[1])
+        blk_max = {}
+        for (jj, d, sc, r, mode) in row:
+            b = (d - 1) // 64
+            q = sc >= cutoff
+            adm = q and _allow(cm, pass_any, local, jj - d + 1, jj, i0, j0, mode, r)
+            prev = blk_max.get(b, -np.inf)
+            if adm and sc > prev:
+                kept.append((jj, d, sc, r, mode))
+            if adm:
+                blk_max[b] = max(prev, sc)
+
+    def replay(cs):
+        arr = (CAND * len(cs))()
+        for a, k in enumerate(rng.permutation(len(cs))):
+            j, d, sc, r, mode = cs[k]
+            arr[a].j, arr[a].d, arr[a].sc, arr[a].r, arr[a].mode = j, d, sc, r, mode
+        th = lib.cm_tophits_Create()
+        eb = C.create_string_buffer(128)
+        st = f(C.cast(cm.ptr, C.c_void_p), eb, pass_idx, 1, C.cast(arr, C.c_void_p), len(cs), None, i0, j0, W, cutoff, 0, 1,
+               C.cast(th, C.c_void_p))
+        assert st == 0, eb.value
+        out = sorted(ia._hits_list(th))
+        lib.cm_tophits_Destroy(th)
+        return out
+
+    everything, filtered = replay(cells), replay(kept)
+    assert len(kept) < len(cells) // 4
+    assert len(everything) > 0
+    assert filtered == everything

@@ -45,7 +45,7 @@ TSTATE = np.dtype([("kind", "<i4"), ("emitmode", "<i4"), ("cnum", "<i4"), ("dx",
                    ("kdx_y", "<i4"), ("kdx_w", "<i4"), ("endsc", "<u4"), ("tsc", "<u4", 6),
                    ("child_base", "<i4", 6), ("ring_mask", "<i4"), ("ring_stride", "<i4"), ("level", "<i4"),
                    ("pad", "<i4")])
-TRPEN = np.dtype([("y", "<i4"), ("base", "<i4"), ("tbase", "<i4"), ("dx", "<i4"), ("is_b", "<i4"), ("pen", "<u4")])
+TRPEN = np.dtype([("y", "<i4"), ("base", "<i4"), ("tbase", "<i4"), ("dx", "<i4"), ("flags", "<i4"), ("pen", "<u4")])
 assert TSTATE.itemsize == 128 and TRPEN.itemsize == 24
 
 
@@ -233,7 +233,7 @@ def replay(cm, p, states=True):
             if ds.size:
                 chk.cells("row", y, ds, "(root begin %d)" % y)
                 chk.cells("row", y, 1, "(root begin %d, lanes past its band)" % y)
-                if q["is_b"]:
+                if int(q["flags"]) & 1:
                     assert int(q["tbase"]) == int(p["st"][y]["tbase"])
                     chk.tcells(y, ds, "(root begin %d)" % y)
     # deck rings of different BEGL states do not overlap
