@@ -444,9 +444,13 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
     const T e0 = pick_esc<S>(ev, xl0);
     const T e1 = pick_esc<S>(ev, xl1);
     const int cb0 = R::tob(carry0), cb1 = R::tob(carry1);
-    T x0 = R::add(R::op(pre0, R::add(lane == 0 ? carry0 : R::imp(), t0), tbl), e0);
-    T x1 = R::add(R::op(pre1, R::add(lane == 0 ? carry1 : R::imp(), t0), tbl), e1);
-    for (int it = 0; it < 65; it++) {
+    /* the start is only a guess: the sweeps run until every lane is a fixed
+     * point of the exact update, which the sequential recursion defines
+     * uniquely (lane k is exact after k + 1 sweeps; the first sweep brings
+     * in the carry) */
+    T x0 = R::add(pre0, e0);
+    T x1 = R::add(pre1, e1);
+    for (int it = 0; it < 66; it++) {
       const T l0 = R::fromb(shr1(cb0, R::tob(x0)));
       const T l1 = R::fromb(shr1(cb1, R::tob(x1)));
       const T n0 = R::add(R::op(pre0, R::add(l0, t0), tbl), e0);
