@@ -390,10 +390,19 @@ def trunc_leg(args, ia, synth, threads):
 
 def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
     """The learned gate on this workload: trained on a separate seeded
-    target (10% emitted homologs) against the plain Inside's own hits, then
-    run on the benchmark target in place of the plain CYK + Inside pass.
-    Plain-method hits: FastFInsideScan hits with E <= 0.01 (the model's
-    local-Inside tail, one strand of the target).  Returns the JSON record."""
+    target (10% emitted homologs), then run on the benchmark target in place
+    of the plain CYK + Inside pass.  Training pairs come from the GPU CYK
+    (--gate-labels cyk, the default): a residue is positive when a
+    FastCYKScan hit scoring at most 5 bits below the CYK score of E = 0.01
+    on the training target (the model's local-CYK tail) covers it, and the
+    gate's threshold is set so the held-out CYK hits of E <= 0.01 pass;
+    --gate-labels inside labels from Inside hits down to T - 5 bits instead.
+    (The pipeline's CYK filter cutoff, P = 1e-4 per hit, lets through random
+    hits covering a few percent of a target: as labels they leave the gate
+    nothing to reject.)  Judged
+    against the plain method's hits: FastFInsideScan hits with E <= 0.01
+    (the model's local-Inside tail, one strand of the target).  Returns the
+    JSON record."""
     from infernal_amd import gate as gt
     from infernal_amd.calibrate import read_exp_infos
     exps = read_exp_infos(cm_path)
@@ -405,7 +414,7 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
     cm = ia.CM.read(cm_path)
     cm.configure(local=True)
     W = cm.W
-    # training target and its plain labels (Inside hits down to T - 5 bits)
+    # training target and its labels
     t0 = time.perf_counter()
     ntr = int(args.gate_train_mb * 1e6)
     bg = synth.markov2_background(ntr, gc=0.40, seed=BG_SEED, sample_seed=4242)   # the benchmark's chain, another sample
@@ -413,9 +422,19 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
     frags = synth.homologs(ia.CM.read(cm_path), nh, seed=4243)
     tcodes, _ = synth.plant(bg, frags, seed=4244)
     tdsq = ia.dsq_from_codes(tcodes)
-    r = ia.FastFInsideScan(cm, 2, tdsq, 1, ntr, T - 5.0, want_hits=True)
-    lab = gt.labels_from_hits(ntr, [(h[0], h[1]) for h in r["hits"]])
-    train_hits = [(h[0], h[1], h[4]) for h in r["hits"] if h[4] >= T]
+    if args.gate_labels == "cyk":
+        if "LC" not in exps:
+            return {"skipped": "model has no local-CYK calibration"}
+        Tc = exps["LC"].update_for_dbsize(float(ntr)).score_for_evalue(0.01)
+        r = ia.FastCYKScan(cm, 1, tdsq, 1, ntr, Tc - 5.0, want_hits=True)
+        lab = gt.labels_from_hits(ntr, [(h[0], h[1]) for h in r["hits"]])
+        train_hits = [(h[0], h[1], h[4]) for h in r["hits"] if h[4] >= Tc]
+        label_rule = "FastCYKScan (QDB1) hits >= %.2f bits (CYK E = 0.01 on the training target: %.2f)" % (Tc - 5.0, Tc)
+    else:
+        r = ia.FastFInsideScan(cm, 2, tdsq, 1, ntr, T - 5.0, want_hits=True)
+        lab = gt.labels_from_hits(ntr, [(h[0], h[1]) for h in r["hits"]])
+        train_hits = [(h[0], h[1], h[4]) for h in r["hits"] if h[4] >= T]
+        label_rule = "FastFInsideScan (QDB2) hits >= %.2f bits" % (T - 5.0)
     t_data = time.perf_counter() - t0
     # training
     net = gt.GateNet(seed=7)
@@ -458,9 +477,59 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
             "gate_score_s": round(t_score, 3), "gated_inside_s": round(t_inside, 3), "windows": len(wins),
             "residue_fraction": round(sum(j - i + 1 for i, j in wins) / n, 5), "tau": round(tau, 3),
             "train": {"target_mb": ntr / 1e6, "homologs": nh, "steps": args.gate_steps, "seconds": round(t_train, 1),
-                      "label_seconds": round(t_data, 1), "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),
+                      "label_seconds": round(t_data, 1), "labels": label_rule, "positive_fraction": round(pos, 5),
+                      "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),
                       "train_hits": len(train_hits), "heldout_hits": len(val), "smooth": args.gate_smooth,
                       "margin": args.gate_margin, "quantile": args.gate_quantile}}
+
+
+def gate_1500_leg(args, ia, synth):
+    """The learned gate on the 1,500-nt synthetic model (the truncated
+    workload's model; E-value calibrated): an inference target of
+    --gate5-mb Mb (2nd-order Markov background, 10% of its residues in
+    emitted homologs), the plain method over all of it (FastCYKScan QDB1 +
+    FastFInsideScan QDB2, timed, target resident in device memory), then the
+    gate trained on --gate5-train-mb Mb of CYK-labelled residues (one
+    (window, CYK label) pair per residue) and run in place of the plain
+    pass.  Judged on the plain FastFInsideScan hits with E <= 0.01."""
+    import copy
+    from infernal_amd.calibrate import read_exp_infos
+    path = os.path.join(ROOT, "tests", "golden", "cm", "synth_nested1500.cm")
+    exps = read_exp_infos(path)
+    if "LI" not in exps or "LC" not in exps:
+        return {"skipped": "synth_nested1500.cm has no E-value calibration"}
+    cm = ia.CM.read(path)
+    cm.configure(local=True)
+    n = int(args.gate5_mb * 1e6)
+    t0 = time.perf_counter()
+    bg = synth.markov2_background(n, gc=0.40, seed=BG_SEED + 5)
+    nh = max(1, int(0.10 * n / (0.75 * cm.clen)))
+    frags = synth.homologs(ia.CM.read(path), nh, seed=5151)
+    codes, _ = synth.plant(bg, frags, seed=5152)
+    dsq = ia.dsq_from_codes(codes)
+    log("gate, 1500-nt model: %.0f-Mb target with %d homologs generated in %.1f s" % (n / 1e6, nh, time.perf_counter() - t0))
+    T = exps["LI"].update_for_dbsize(float(n)).score_for_evalue(0.01)
+    ia.pin_target(dsq)
+    try:
+        ia.FastCYKScan(cm, 1, dsq, 1, min(n, 4000), 0.0)          # first-use set-up
+        ia.FastFInsideScan(cm, 2, dsq, 1, min(n, 4000), 0.0)
+        t0 = time.perf_counter()
+        ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
+        t_cyk = time.perf_counter() - t0
+        log("gate, 1500-nt model: plain CYK %.1f s" % t_cyk)
+        t0 = time.perf_counter()
+        ri = ia.FastFInsideScan(cm, 2, dsq, 1, n, T, want_hits=True)
+        t_ins = time.perf_counter() - t0
+        plain_hits = [(h[0], h[1], h[4]) for h in ri["hits"]]
+        log("gate, 1500-nt model: plain Inside %.1f s, %d hits with E <= 0.01 (T = %.2f bits)" % (t_ins, len(plain_hits), T))
+        a5 = copy.copy(args)
+        a5.gate_train_mb, a5.gate_steps = args.gate5_train_mb, args.gate5_steps
+        gate = gate_leg(a5, ia, synth, path, codes, dsq, plain_hits, t_cyk + t_ins)
+    finally:
+        ia.unpin_target()
+    gate.update({"cm": "%s (clen %d, M %d, W %d)" % (cm.c.name.decode(), cm.clen, cm.M, cm.W), "target_mb": n / 1e6,
+                 "homologs": nh, "plain_cyk_s": round(t_cyk, 2), "plain_inside_s": round(t_ins, 2)})
+    return gate
 
 
 def _cells(cm, qdbidx, i0, j0, truncated=False):
@@ -528,6 +597,8 @@ def main():
     ap.add_argument("--gate-smooth", type=int, default=128, help="residues the gate's logits are averaged over")
     ap.add_argument("--gate-margin", type=float, default=0.5, help="logit margin below the threshold quantile")
     ap.add_argument("--gate-quantile", type=float, default=0.005, help="held-out hit quantile the threshold sits at")
+    ap.add_argument("--gate-labels", choices=("cyk", "inside"), default="cyk",
+                    help="training labels: GPU CYK hits at the CYK filter cutoff, or Inside hits")
     ap.add_argument("--cyk1500-mb", type=float, default=0.0,
                     help="also scan this many Mb with the 1,500-nt model (FastCYKScan QDB1), GPU and CPU; 0 = skip")
     ap.add_argument("--trunc-windows", type=int, default=0,
@@ -545,6 +616,11 @@ def main():
     ap.add_argument("--batch-tune", action="store_true", help="many-models leg: also run with autotuned launch parameters")
     ap.add_argument("--batch-only", action="store_true", help="run only the many-models leg and print its JSON")
     ap.add_argument("--latency-only", action="store_true", help="run only the single-call latency leg and print its JSON")
+    ap.add_argument("--gate5-only", action="store_true", help="run only the learned gate on the 1,500-nt model and print its JSON")
+    ap.add_argument("--cyk1500-only", action="store_true", help="run only the 1,500-nt CYK leg (--cyk1500-mb, default 20) and print its JSON")
+    ap.add_argument("--gate5-mb", type=float, default=20.0, help="1,500-nt gate leg: inference target in megabases")
+    ap.add_argument("--gate5-train-mb", type=float, default=50.0, help="1,500-nt gate leg: CYK-labelled training target in megabases")
+    ap.add_argument("--gate5-steps", type=int, default=6000, help="1,500-nt gate leg: training steps")
     ap.add_argument("--latency-reps", type=int, default=5, help="calls per latency case (median reported)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device-to-device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -583,6 +659,14 @@ def main():
         return
     if args.latency_only:
         print(json.dumps(latency_leg(args, ia, synth)))
+        return
+    if args.gate5_only:
+        print(json.dumps(gate_1500_leg(args, ia, synth)))
+        return
+    if args.cyk1500_only:
+        if args.cyk1500_mb <= 0:
+            args.cyk1500_mb = 20.0
+        print(json.dumps(cyk_1500(args, ia, synth, None if args.no_cpu else threads)))
         return
 
     cm = ia.CM.read(args.cm)

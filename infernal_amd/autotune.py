@@ -86,7 +86,10 @@ def tune(algo, cms, qdbidx, dsq, windows, cutoffs, sample=0.1, seed=0, run=None,
     trials = []
 
     def timed(cfg):
-        t = min(run(cfg, sub) for _ in range(reps))
+        try:
+            t = min(run(cfg, sub) for _ in range(reps))
+        except ia.ScanError:
+            t = float("inf")                 # the shape does not fit this workload (LDS, registers)
         trials.append((dict(cfg), t))
         return t
 

@@ -22,6 +22,7 @@ The random stream differs from Easel's, so calibrated values agree with the
 shipped ECM lines statistically, not digit for digit.
 """
 import math
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -167,12 +168,18 @@ def sample_genomic(nseq, L, rng):
     return out
 
 
-def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk=10000, null3=True, joined=False):
+def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk=10000, null3=True, joined=False,
+              progress=None):
     """cmcalibrate's exponential tails for the model in cm_path, the scans
     on the GPU.  Returns {mode: ExpInfo} and the hit scores per mode.
     joined: scan the generated pieces end to end as one target (one launch
     per mode instead of one per piece; hits may then span the junctions
-    between pieces, a few in a thousand for W of a few hundred)."""
+    between pieces, a few in a thousand for W of a few hundred).
+    null3: score hits with the NULL3 correction, as cmcalibrate does.  The
+    scans report every hit (cutoff -inf, src/cmcalibrate.c:1845); with NULL3
+    every banded root cell goes to the host replay (about L * W of them),
+    without it the scanners' greedy record filter keeps a few per row.
+    progress(mode, seconds, hits) is called after each mode."""
     from . import CM, FastCYKScan, FastIInsideScan, dsq_from_codes
     nseq = int(L_mb * 1e6 / chunk + 0.5)
     total = nseq * chunk
@@ -184,6 +191,7 @@ def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk
     dsqs = [dsq_from_codes(s) for s in seqs]
     res, all_scores = {}, {}
     for mode in modes:
+        t0 = time.perf_counter()
         cm = CM.read(cm_path)
         cm.configure(local=(mode[0] == "L"))
         scan = FastIInsideScan if mode[1] == "I" else FastCYKScan
@@ -195,4 +203,6 @@ def calibrate(cm_path, L_mb=1.6, seed=181, modes=("GC", "GI", "LC", "LI"), chunk
         mu, lam, _ = fit_exp_tail(scores, tp)
         res[mode] = ExpInfo.from_fit(lam, mu, float(total), len(scores), tp)
         all_scores[mode] = np.asarray(scores)
+        if progress:
+            progress(mode, time.perf_counter() - t0, len(scores))
     return res, all_scores

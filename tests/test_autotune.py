@@ -51,3 +51,18 @@ def test_apply_sets_and_restores_overrides():
         assert "INFX_WIDE" not in os.environ
     assert "INFX_BEGIN_ACC" not in os.environ and os.environ.get("INFX_WIDE") == "0"
     os.environ.pop("INFX_WIDE", None)
+
+
+def test_candidates_that_do_not_fit_are_skipped():
+    """A launch shape the scanner refuses (ScanError: LDS or registers do not
+    fit the workload) is recorded as infinitely slow, and the search goes on."""
+    def run(cfg, ws):
+        if cfg["wpb"] == 16:
+            raise ia.ScanError(7, "no workgroup shape fits")
+        return {0: 3.0, 8: 1.0, 4: 2.0}[cfg["wpb"]] + {0: 2.0, 2: 1.0, 4: 3.0, 8: 4.0}[cfg["chunk_w"]]
+
+    cms = _cms()
+    wins = [(1 + 300 * k, 250 + 300 * k, k % 2) for k in range(30)]
+    best, trials = autotune.tune("cyk", cms, 1, None, wins, [0.0, 0.0], sample=0.5, run=run)
+    assert best["wpb"] == 8 and best["chunk_w"] == 2
+    assert [t for c, t in trials if c["wpb"] == 16] == [float("inf")]
