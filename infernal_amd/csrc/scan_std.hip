@@ -282,13 +282,20 @@ __device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bo
 /* NB consecutive 64-cell blocks of a non-B, non-IL state, rows j (0) and
  * j+1 (1), from d0: all loads first (2*NB independent cells per lane), then
  * the combines, then the stores.  Lanes past a row's band compute a copy of
- * a valid cell and store nothing. */
-template <int S, bool VSC, int CN, bool BEGL, int NB>
+ * a valid cell and store nothing.
+ * IR (the state is its own first child on the j-1 row, at d-1): row j+1's
+ * self term is row j's cell d-1, which this visit computes.  It is taken
+ * from row j's results in registers, shifted one lane (DPP wave_shr:1; lane 0
+ * from `irc`, row j's cell d0-1: the previous block's lane 63, or the stored
+ * cell dn-1 that no visit writes), so both rows are one pass.  combine_children
+ * applies the self term last, as the reference's order for each child count
+ * does. */
+template <int S, bool VSC, int CN, bool BEGL, int NB, bool IR = false>
 __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
                                             const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
                                             typename SR<S>::T *out0, typename SR<S>::T *out1, const int *cb,
                                             int d0, int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                            int ev0, int ev1, bool vs0, bool vs1)
+                                            int ev0, int ev1, bool vs0, bool vs1, typename SR<S>::T &irc)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -314,14 +321,14 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
 #pragma unroll
     for (int k = 0; k < CN; k++) {
       c0[b][k] = ldu(crow0, (uint32_t) (cb[k] + dy0[b]));
-      c1[b][k] = ldu(crow1, (uint32_t) (cb[k] + dy1[b]));
+      c1[b][k] = (IR && k == 0) ? R::imp() : ldu(crow1, (uint32_t) (cb[k] + dy1[b]));
     }
   }
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     hold(xl0[b]); hold(xl1[b]);
 #pragma unroll
-    for (int k = 0; k < CN; k++) { hold(c0[b][k]); hold(c1[b][k]); }
+    for (int k = 0; k < CN; k++) { hold(c0[b][k]); if (!(IR && k == 0)) hold(c1[b][k]); }
   }
   /* right residue's score (EMITRIGHT): read after the loads are issued */
   const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
@@ -337,6 +344,13 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
       sc0[b] = ini0; sc1[b] = ini1;
 #pragma unroll
       for (int k = 0; k < CN; k++) { sc0[b] = R::op(sc0[b], c0[b][k], tbl); sc1[b] = R::op(sc1[b], c1[b][k], tbl); }
+    } else if (IR) {
+      sc0[b] = combine_children<S, CN>(c0[b], ini0, true, tbl);
+      if (em != EMITNONE) sc0[b] = R::add(sc0[b], (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0[b]));
+      c1[b][0] = R::add(R::fromb(shr1(R::tob(irc), R::tob(sc0[b]))), R::bits(s.tsc[0]));
+      irc = R::fromb(__builtin_amdgcn_readlane(R::tob(sc0[b]), 63));
+      sc1[b] = combine_children<S, CN>(c1[b], ini1, true, tbl);
+      if (em != EMITNONE) sc1[b] = R::add(sc1[b], (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1[b]));
     } else {
       sc0[b] = combine_children<S, CN>(c0[b], ini0, true, tbl);
       sc1[b] = combine_children<S, CN>(c1[b], ini1, true, tbl);
@@ -370,7 +384,7 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
  * per SIMD anyway, and the wide-band kernels of models whose targets cannot
  * fill the chip with wavefronts.  Returns the max score over the rows
  * flagged in vs. */
-template <int S, bool VSC, int CN, bool BEGL, int NBW>
+template <int S, bool VSC, int CN, bool BEGL, int NBW, bool IR = false>
 __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
                                             const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
                                             typename SR<S>::T *out0, typename SR<S>::T *out1,
@@ -384,19 +398,60 @@ __device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
   const int dxm = max(dx0, dx1);
   float vmax = IMPF;
+  /* IR: row j's cell dn-1 (outside the band, never written by a visit) */
+  T irc = IR ? ldu(crow1, (uint32_t) (cb[0] + dn - 1)) : R::imp();
   int d0 = dn;
   if (NBW >= 4)
     for (; d0 + 192 <= dxm; d0 += 256)
-      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 4>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 4, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                             hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
   if (NBW >= 2)
     for (; d0 + 64 <= dxm; d0 += 128)
-      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                         hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                             hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
   for (; d0 <= dxm; d0 += 64)
-    vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                       hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1));
+    vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
+                                                           hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
   return vmax;
+}
+
+/* FInside IL chains: a starting point for the fixed-point sweep.  In the
+ * linear domain the chain x_d = log2(2^pre_d + 2^(x_{d-1} + t0)) + e_d is the
+ * affine recurrence X_d = 2^(t0+e_d) X_{d-1} + 2^(pre_d+e_d); its maps compose
+ * associatively, (A2, B2) o (A1, B1) = (A2 + A1, lse(B2, A2 + B1)) in log2
+ * terms, so a Kogge-Stone prefix over the 64 lanes (DPP row_shr 1, 2, 4, 8
+ * within each 16-lane row, then row_bcast 15 and 31 across rows) gives every
+ * lane its map from the carry x_{d0-1}.  lse uses the hardware's exp2/log2,
+ * not the reference's table, so the result is only close to the reference's
+ * values (within a few table steps); the sweeps that follow make every lane
+ * exact.  Lanes without a source take the identity (A 0, B far below any DP
+ * value), which leaves their map unchanged bit for bit. */
+__device__ __forceinline__ float lse2_hw(float x, float y)
+{
+  const float m = fmaxf(x, y);
+  const float t = __builtin_amdgcn_exp2f(-fabsf(__fsub_rn(x, y)));
+  return __fadd_rn(m, __builtin_amdgcn_logf(__fadd_rn(1.0f, t)));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void il_scan_step(float &A, float &B)
+{
+  const float oA = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(0.0f), __float_as_int(A), CTRL, ROWS, 0xf, false));
+  const float oB = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(-1e37f), __float_as_int(B), CTRL, ROWS, 0xf, false));
+  B = lse2_hw(B, __fadd_rn(A, oB));
+  A = __fadd_rn(A, oA);
+}
+
+__device__ __forceinline__ float il_guess_finside(float pre, float e, float t0, float carry)
+{
+  float A = __fadd_rn(t0, e), B = __fadd_rn(pre, e);
+  il_scan_step<0x111, 0xf>(A, B);   /* row_shr:1 */
+  il_scan_step<0x112, 0xf>(A, B);   /* row_shr:2 */
+  il_scan_step<0x114, 0xf>(A, B);   /* row_shr:4 */
+  il_scan_step<0x118, 0xf>(A, B);   /* row_shr:8 */
+  il_scan_step<0x142, 0xa>(A, B);   /* row_bcast:15 into rows 1 and 3 */
+  il_scan_step<0x143, 0xc>(A, B);   /* row_bcast:31 into rows 2 and 3 */
+  return lse2_hw(B, __fadd_rn(A, carry));
 }
 
 /* IL state: x_d = op(pre_d, x_{d-1} + t0) + e_d with pre_d the other CN-1
@@ -446,10 +501,17 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
     const int cb0 = R::tob(carry0), cb1 = R::tob(carry1);
     /* the start is only a guess: the sweeps run until every lane is a fixed
      * point of the exact update, which the sequential recursion defines
-     * uniquely (lane k is exact after k + 1 sweeps; the first sweep brings
-     * in the carry) */
-    T x0 = R::add(pre0, e0);
-    T x1 = R::add(pre1, e1);
+     * uniquely (lane k is exact after k + 1 sweeps from any start).  CYK and
+     * IInside start from pre + e (the first sweep brings in the carry);
+     * FInside from the prefix-scan estimate of the whole chain. */
+    T x0, x1;
+    if ((S & 7) == SEMI_FINSIDE) {
+      x0 = il_guess_finside(pre0, e0, t0, carry0);
+      x1 = il_guess_finside(pre1, e1, t0, carry1);
+    } else {
+      x0 = R::add(pre0, e0);
+      x1 = R::add(pre1, e1);
+    }
     for (int it = 0; it < 66; it++) {
       const T l0 = R::fromb(shr1(cb0, R::tob(x0)));
       const T l1 = R::fromb(shr1(cb1, R::tob(x1)));
@@ -718,23 +780,12 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         const T *crow1 = rd ? cur0 : cur1;
         float vmax;
         if (kind == KIND_GENERIC) {
-          /* IR reads its own row j cells for row j+1: finish row j first */
+          /* IR reads its own row j cells for row j+1: gen_blocks<IR> takes
+           * them from row j's results in registers */
           const bool seq = (hdr >> 14) & 1;
-          const int gx1 = seq ? dn - 1 : dx1;
-          switch (cnum) {
-#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, gx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
-          case 2:  GEN(2); break;
-          case 3:  GEN(3); break;
-          case 4:  GEN(4); break;
-          case 5:  GEN(5); break;
-          default: GEN(6); break;
-#undef GEN
-          }
-          if (seq && dx1 >= dn) {
-            wave_sync();
-            float vm1;
+          if (seq) {
             switch (cnum) {
-#define GEN(CN) vm1 = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dn - 1, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW, true>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
             case 2:  GEN(2); break;
             case 3:  GEN(3); break;
             case 4:  GEN(4); break;
@@ -742,7 +793,16 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
             default: GEN(6); break;
 #undef GEN
             }
-            vmax = fmaxf(vmax, vm1);
+          } else {
+            switch (cnum) {
+#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+            case 2:  GEN(2); break;
+            case 3:  GEN(3); break;
+            case 4:  GEN(4); break;
+            case 5:  GEN(5); break;
+            default: GEN(6); break;
+#undef GEN
+            }
           }
         } else if (kind == KIND_IL) {
           switch (cnum) {

@@ -125,8 +125,12 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
     cb = [cbase + s16(int(w[13 + (k >> 1)]) >> (16 * (k & 1))) for k in range(6)]
     if kind in (KIND_GENERIC, KIND_BEGL):
         rmask, rstride = aux & 0xffff, aux >> 16
-        passes = [(dx0, dx1)] if not (kind == KIND_GENERIC and seq) else [(dx0, dn - 1), (dn - 1, dx1)]
-        for (x0, x1) in passes:
+        ir = kind == KIND_GENERIC and seq
+        if ir:
+            # row j+1's self term comes from row j in registers; lane 0 of the
+            # first block from the stored cell dn-1 of row j
+            chk.row(np.array([cb[0] + dn - 1]), ctx + " ir carry")
+        for (x0, x1) in [(dx0, dx1)]:
             for d0 in range(dn, max(x0, x1) + 1, 64):
                 d = d0 + LANES
                 a0, a1 = d <= x0, d <= x1
@@ -136,7 +140,8 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
                 chk.dsq(dsq1(np.where(a1, d, 1)), ctx + " xl1")
                 for k in range(cnum):
                     chk.row(cb[k] + dy0, ctx + " child0")
-                    chk.row(cb[k] + dy1, ctx + " child1")
+                    if not (ir and k == 0):
+                        chk.row(cb[k] + dy1, ctx + " child1")
                 if kind == KIND_BEGL:
                     chk.deck(((jp0 & rmask) * rstride + base + d)[a0], ctx + " begl out0")
                     chk.deck(((jp1 & rmask) * rstride + base + d)[a1], ctx + " begl out1")
