@@ -771,7 +771,7 @@ def main():
         if os.path.exists(tj):
             with open(tj) as f:
                 tr = json.load(f)
-            if tr.get("cm") == cm.c.name.decode() and tr.get("block_residues") == blocks[1][1] - blocks[1][0] + 1:
+            if tr.get("cm") == cm.c.name.decode() and tr.get("block_residues") == blocks[0][1] - blocks[0][0] + 1:
                 roof["traffic"] = tr["hbm_bytes_per_launch"]
                 roof["traffic_note"] = tr.get("note", "")
                 roof["traffic_rate"] = traffic_rate(tr["hbm_bytes_per_launch"], kms_avg)

@@ -19,6 +19,8 @@
 extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int gacc, int wide, dim3 grid, dim3 block,
                                       hipStream_t stream, const ScanArgs *a, int acc_n);
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc);
+extern "C" size_t infx_std_batch_lds_bytes(int semi, int wpb, int acc_n);
+extern "C" int infx_std_rows(int semi, int tbl_lds, int wide);
 extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
 extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n);
 extern "C" hipError_t infx_launch_zero_u64(unsigned long long *p, hipStream_t stream);
@@ -403,8 +405,11 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
   if (semi == SEMI_IINSIDE) d0_init_int(cm, a0i); else d0_init_float(cm, a0f);
   const uint32_t impbits = (semi == SEMI_IINSIDE) ? (uint32_t) (-INFTY) : fbits((float) IMPOSSIBLE);
 
-  /* BEGL decks: a ring per BEGL state, as many rows as the largest split
-   * k its B parent uses (+1), rounded up to a power of two */
+  /* BEGL decks: a ring per BEGL state, rounded up to a power of two.  A
+   * state visit writes rows jp .. jp+NR-1 of the deck before its B parent
+   * reads rows jp+t-k for k up to the largest split maxk, so rows jp-maxk ..
+   * jp+NR-1 must not share a ring row: maxk + NR rows (NR = 4, the most
+   * rows any kernel visits at once) */
   std::vector<int> ring(M, 0);
   for (int v = 1; v < M; v++) {
     if (cm->sttype[v] != B_st || dn[v] > dx[v]) continue;
@@ -414,7 +419,7 @@ static int build_std_profile(CM_t *cm, CM_SCAN_MX *smx, int qdbidx, int semi, in
       if (kmin <= kmax) maxk = ESL_MAX(maxk, kmax);
     }
     int r = 1;
-    while (r < maxk + 1) r <<= 1;
+    while (r < maxk + INFX_NR_MAX) r <<= 1;
     ring[w] = r;
   }
 
@@ -591,15 +596,16 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   StdProfile fresh;
   if ((status = build_std_profile(cm, smx, qdbidx, semi, W, fresh, errbuf)) != eslOK) return status;
 
-  /* DP slot of one wavefront: guard | deck | guard | rows[4] (ring over j) |
-   * guard | vsc | global accumulators (rows j, j+1; 4-byte cells) */
-  auto slot_layout = [&](int R, int RB, int gacc_, int acc_n_, size_t *deck_off_, size_t *row_off_, size_t *vsc_off_, size_t *acc_off_) {
+  /* DP slot of one wavefront: guard | deck | guard | rows[2 NR] (ring over j) |
+   * guard | vsc | global accumulators (rows j .. j+NR-1; 4-byte cells), with
+   * NR the rows per state visit of the kernel */
+  auto slot_layout = [&](int R, int RB, int gacc_, int acc_n_, int nr_, size_t *deck_off_, size_t *row_off_, size_t *vsc_off_, size_t *acc_off_) {
     const size_t G = (size_t) ((W + 128 + 63) / 64) * 64;
     *deck_off_ = G * 4;
     *row_off_  = *deck_off_ + ((size_t) RB + G) * 4;
-    *vsc_off_  = *row_off_ + ((size_t) 4 * R + G) * 4;
+    *vsc_off_  = *row_off_ + ((size_t) 2 * nr_ * R + G) * 4;
     *acc_off_  = ((*vsc_off_ + (size_t) cm->M * 4) + 255) / 256 * 256;
-    return ((*acc_off_ + (gacc_ ? (size_t) acc_n_ * 8 : 0)) + 255) / 256 * 256;
+    return ((*acc_off_ + (gacc_ ? (size_t) acc_n_ * 4 * nr_ : 0)) + 255) / 256 * 256;
   };
   auto r256 = [](size_t n) { return (n + 255) / 256 * 256; };
   /* the no-allocation path: everything must fit the caller's fixed region
@@ -614,7 +620,8 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
                         (pinned ? 256 : r256((size_t) Lspan + 128)) + r256(sizeof(ScanArgs)) + 256 +
                         r256(ESL_MAX((size_t) nch_max * 24 + 64, r256((size_t) nch_max * 4) + (size_t) nch_max * 16 + 64)) +
                         (nwin > 1 ? r256((size_t) nch_max * 24) : 0) +
-                        slot_layout(fresh.R, fresh.RB, 0, 0, &d_, &r_, &v_, &a_) + r256((size_t) ar_cand_min * sizeof(Cand));
+                        slot_layout(fresh.R, fresh.RB, 0, 0, infx_std_rows(semi, 1, 0), &d_, &r_, &v_, &a_) +
+                        r256((size_t) ar_cand_min * sizeof(Cand));
     if (need > ar->left())
       return infx_fail(eslERANGE, errbuf, "%s: this model and target need %zu bytes of device scratch; the no-allocation scratch holds %zu",
                        fname, need, ar->left());
@@ -747,7 +754,8 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
       for (int64_t a = wi[w]; a <= wj[w]; a += chunk) { ctab.push_back(a); ctab.push_back(ESL_MIN(wj[w], a + chunk - 1)); ctab.push_back(wi[w]); }
   }
   size_t deck_off, row_off, vsc_off, acc_off;
-  const size_t slot_bytes = slot_layout(P->R, P->RB, gacc, acc_n, &deck_off, &row_off, &vsc_off, &acc_off);
+  const int nrows = infx_std_rows(semi, tbl_lds, wide);
+  const size_t slot_bytes = slot_layout(P->R, P->RB, gacc, acc_n, nrows, &deck_off, &row_off, &vsc_off, &acc_off);
   const size_t chunk_out_bytes = ESL_MAX((size_t) nchunks * 24 + 64, ((((size_t) nchunks * 4 + 255) / 256) * 256) + (size_t) nchunks * 16 + 64);
   size_t scratch_budget;
   if (ar) {
@@ -1147,7 +1155,7 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
   int wpb = 0, occ = 0;
   for (int w = 16; w >= 1; w >>= 1) {
     if (infx_wpb_override() && w != infx_wpb_override()) continue;
-    if (infx_std_lds_bytes(semi, 1, w, acc_max, 0) > (size_t) g_lds_max) continue;
+    if (infx_std_batch_lds_bytes(semi, w, acc_max) > (size_t) g_lds_max) continue;
     const int o = infx_std_batch_occupancy(semi, w, acc_max);
     if (o * w > occ * wpb) { wpb = w; occ = o; }
   }

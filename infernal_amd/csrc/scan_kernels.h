@@ -10,7 +10,8 @@
  *                        chunk position jp is (jp & (ring_w - 1)) and cell d
  *                        is deck[base_w + row * stride_w + d]
  *   guard[G]
- *   row[2][R]            the rolling j / j-1 rows of every other state;
+ *   row[2NR][R]          a ring over j of the rows of every other state (NR
+ *                        rows per state visit: rows j-1 .. j+NR-1 live);
  *                        state v holds cells d = lo_v..hi_v at row[base_v + d]
  *   vsc[M]               per-state running max (only when requested)
  * lo_v..hi_v covers the band and every cell a parent can read, so reads need
@@ -18,7 +19,8 @@
  * (IMPOSSIBLE, or the d = 0 value), exactly as the reference's full
  * alpha[2][M][W+1] would.
  * LDS: per workgroup the log-sum table (Inside), per wavefront the local-begin
- * accumulators acc_sc[acc_n], acc_r[acc_n] for d = root_dn .. root_dn + acc_n - 1.
+ * maxima of the NR rows of a visit, acc_n cells each for d = root_dn ..
+ * root_dn + acc_n - 1 (or in the slot, after vsc, for wide root bands).
  */
 #ifndef INFX_SCAN_KERNELS_H
 #define INFX_SCAN_KERNELS_H
@@ -33,6 +35,7 @@
 #define INFX_LOGSUM_N 23000   /* entries of the FLogsum / ILogsum tables */
 #define INFX_LOGSUM_PAD 23008 /* device copies: entries 23000.. are 0 (FLogsum's 'return max' index) */
 #define INFX_NOALLOC_BYTES ((size_t) 256 << 20)   /* the no-allocation scanner's fixed scratch per GPU */
+#define INFX_NR_MAX 4        /* most rows a standard kernel computes per state visit (BEGL deck rings allow for it) */
 
 enum { KIND_GENERIC = 0, KIND_IL = 1, KIND_B = 2, KIND_BEGL = 3, KIND_SKIP = 4 };
 enum { SEMI_CYK = 0, SEMI_FINSIDE = 1, SEMI_IINSIDE = 2 };

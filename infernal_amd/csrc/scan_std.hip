@@ -248,18 +248,23 @@ __device__ __forceinline__ typename SR<S>::T pick_esc(int ev_bits, int x)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Two-row sweeps.  A wavefront computes rows j and j+1 of a state before it
- * moves to the next state: every cell of row j+1 depends only on higher
- * states (rows j, j+1), the same state's row j+1 (the IL self-loop) and rows
- * <= j, so the order is the reference's order for each row, and the state
- * descriptor, emission row and loop overheads are paid once per two rows.
- * The row storage is a ring of 4 rows (j & 3): rows j-1, j and j+1 live. */
+/* Multi-row sweeps.  A wavefront computes NR consecutive rows j..j+NR-1 of a
+ * state before it moves to the next state.  Row j+t of a state depends only
+ * on higher states (rows j+t and j+t-1), on its own row j+t (the IL
+ * self-loop) and, for IR, on its own row j+t-1, so the order is the
+ * reference's order for each row, while the state descriptor, emission rows
+ * and loop overheads are paid once per NR rows.  The row storage is a ring of
+ * 2*NR rows (j & (2NR-1)): rows j-1 .. j+NR-1 live.  Rows are addressed as
+ * 32-bit cell offsets (ring row * R) from the slot's row base.  NR is 2 for
+ * CYK (8 wavefronts per SIMD leave 64 VGPRs) and 4 for Inside with the LDS
+ * table (one 16-wavefront workgroup per CU: 128 VGPRs). */
 
 /* Per-sweep context passed as plain values (a struct would be selected by
- * field address and spilled to scratch). */
-#define SW_PARAMS const typename SR<S>::T *tbl, const uint8_t *dsq_b0, int dsq_w, int xj0, int xj1, uint32_t el, int lane, \
-                  typename SR<S>::T *acc0, typename SR<S>::T *acc1, int rdn
-#define SW_ARGS   tbl, dsq_b0, dsq_w, xj0, xj1, el, lane, acc0, acc1, rdn
+ * field address and spilled to scratch).  xj[t]: right residue of row j+t;
+ * the local-begin maxima of row j+t are at acc0 + t * acc_n. */
+#define SW_PARAMS const typename SR<S>::T *tbl, const uint8_t *dsq_b0, int dsq_w, const int *xj, uint32_t el, int lane, \
+                  typename SR<S>::T *acc0, int acc_n, int rdn
+#define SW_ARGS   tbl, dsq_b0, dsq_w, xj, el, lane, acc0, acc_n, rdn
 
 /* local-begin candidate of cell d (src/cm_dpsearch.c:599-612): the largest
  * alpha + beginsc over begin states; the state is found again only for
@@ -279,23 +284,26 @@ __device__ __forceinline__ void begin_update(typename SR<S>::T *acc, int rdn, bo
   }
 }
 
-/* NB consecutive 64-cell blocks of a non-B, non-IL state, rows j (0) and
- * j+1 (1), from d0: all loads first (2*NB independent cells per lane), then
- * the combines, then the stores.  Lanes past a row's band compute a copy of
- * a valid cell and store nothing.
- * IR (the state is its own first child on the j-1 row, at d-1): row j+1's
- * self term is row j's cell d-1, which this visit computes.  It is taken
- * from row j's results in registers, shifted one lane (DPP wave_shr:1; lane 0
- * from `irc`, row j's cell d0-1: the previous block's lane 63, or the stored
- * cell dn-1 that no visit writes), so both rows are one pass.  combine_children
- * applies the self term last, as the reference's order for each child count
- * does. */
-template <int S, bool VSC, int CN, bool BEGL, int NB, bool IR = false>
+/* NB consecutive 64-cell blocks of a non-B, non-IL state, rows j+t for
+ * t < NR, from d0: all loads first (NR*NB independent cells per child and
+ * lane), then the combines, then the stores.  Children are read at
+ * ibase[ioff[t] + ...] (row j+t, or j+t-1 for right emitters), results
+ * stored at obase[ooff[t] + ...] (the row ring, or a BEGL deck ring).  Lanes
+ * past a row's band (dx[t]; a row past the chunk has dx < dn) compute a copy
+ * of a valid cell and store nothing.
+ * IR (the state is its own first child on the j-1 row, at d-1): row j+t's
+ * self term (t >= 1) is row j+t-1's cell d-1, which this visit computes.  It
+ * is taken from that row's results in registers, shifted one lane (DPP
+ * wave_shr:1; lane 0 from irc[t], row j+t-1's cell d0-1: the previous
+ * block's lane 63, or the stored cell dn-1 that no visit writes).
+ * combine_children applies the self term last, as the reference's order for
+ * each child count does. */
+template <int S, bool VSC, int CN, bool BEGL, int NB, bool IR, int NR>
 __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
-                                            const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
-                                            typename SR<S>::T *out0, typename SR<S>::T *out1, const int *cb,
-                                            int d0, int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                            int ev0, int ev1, bool vs0, bool vs1, typename SR<S>::T &irc)
+                                            const typename SR<S>::T *ibase, const int *ioff,
+                                            typename SR<S>::T *obase, const int *ooff, const int *cb,
+                                            int d0, int dn, const int *dx, bool hb, const int *bdx,
+                                            const int *ev, const bool *vs, typename SR<S>::T *irc)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -304,114 +312,106 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
   const int init_ok = (hdr >> 12) & 1;
   const T bsc = R::bits(s.beginsc);
   const int bdn = max(rdn, dn);
-  const uint8_t *dsq_b1 = dsq_b0 + 1;
   float vmax = IMPF;
-  int d[NB], dy0[NB], dy1[NB], xl0[NB], xl1[NB];
-  bool a0[NB], a1[NB];
-  T c0[NB][CN], c1[NB][CN];
+  int d[NB], dy[NR][NB], xl[NR][NB];
+  bool a[NR][NB];
+  T c[NR][NB][CN];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     d[b] = d0 + 64 * b + lane;
-    a0[b] = d[b] <= dx0; a1[b] = d[b] <= dx1;
-    dy0[b] = (a0[b] ? d[b] : dn) - sd; dy1[b] = (a1[b] ? d[b] : dn) - sd;   /* idle lanes read a valid cell */
-    /* left residue j-d+1 (idle lanes: residue j, inside the target even where
-     * the row's band is still empty at the start of a chunk) */
-    xl0[b] = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0[b] ? d[b] : 1)));
-    xl1[b] = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1[b] ? d[b] : 1)));
 #pragma unroll
-    for (int k = 0; k < CN; k++) {
-      c0[b][k] = ldu(crow0, (uint32_t) (cb[k] + dy0[b]));
-      c1[b][k] = (IR && k == 0) ? R::imp() : ldu(crow1, (uint32_t) (cb[k] + dy1[b]));
+    for (int t = 0; t < NR; t++) {
+      a[t][b] = d[b] <= dx[t];
+      dy[t][b] = (a[t][b] ? d[b] : dn) - sd;   /* idle lanes read a valid cell */
+      /* left residue j+t-d+1 (idle lanes: residue j+t, inside the target even
+       * where the row's band is still empty at the start of a chunk) */
+      xl[t][b] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t][b] ? d[b] : 1)));
+#pragma unroll
+      for (int k = 0; k < CN; k++)
+        c[t][b][k] = (IR && t > 0 && k == 0) ? R::imp() : ldu(ibase, (uint32_t) (ioff[t] + cb[k] + dy[t][b]));
     }
   }
 #pragma unroll
-  for (int b = 0; b < NB; b++) {
-    hold(xl0[b]); hold(xl1[b]);
+  for (int b = 0; b < NB; b++)
 #pragma unroll
-    for (int k = 0; k < CN; k++) { hold(c0[b][k]); if (!(IR && k == 0)) hold(c1[b][k]); }
-  }
+    for (int t = 0; t < NR; t++) {
+      hold(xl[t][b]);
+#pragma unroll
+      for (int k = 0; k < CN; k++) if (!(IR && t > 0 && k == 0)) hold(c[t][b][k]);
+    }
   /* right residue's score (EMITRIGHT): read after the loads are issued */
-  const T er0 = R::fromb(__builtin_amdgcn_readlane(ev0, xj0));
-  const T er1 = R::fromb(__builtin_amdgcn_readlane(ev1, xj1));
-  T sc0[NB], sc1[NB];
+  T er[NR];
+#pragma unroll
+  for (int t = 0; t < NR; t++) er[t] = R::fromb(__builtin_amdgcn_readlane(ev[t], xj[t]));
+  T sc[NR][NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
 #pragma unroll
-    for (int k = 0; k < CN; k++) { c0[b][k] = R::add(c0[b][k], R::bits(s.tsc[k])); c1[b][k] = R::add(c1[b][k], R::bits(s.tsc[k])); }
-    const T ini0 = R::init(init_ok, el, s.endsc, dy0[b]);
-    const T ini1 = R::init(init_ok, el, s.endsc, dy1[b]);
-    if (BEGL) {
-      sc0[b] = ini0; sc1[b] = ini1;
+    for (int t = 0; t < NR; t++) {
 #pragma unroll
-      for (int k = 0; k < CN; k++) { sc0[b] = R::op(sc0[b], c0[b][k], tbl); sc1[b] = R::op(sc1[b], c1[b][k], tbl); }
-    } else if (IR) {
-      sc0[b] = combine_children<S, CN>(c0[b], ini0, true, tbl);
-      if (em != EMITNONE) sc0[b] = R::add(sc0[b], (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0[b]));
-      c1[b][0] = R::add(R::fromb(shr1(R::tob(irc), R::tob(sc0[b]))), R::bits(s.tsc[0]));
-      irc = R::fromb(__builtin_amdgcn_readlane(R::tob(sc0[b]), 63));
-      sc1[b] = combine_children<S, CN>(c1[b], ini1, true, tbl);
-      if (em != EMITNONE) sc1[b] = R::add(sc1[b], (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1[b]));
-    } else {
-      sc0[b] = combine_children<S, CN>(c0[b], ini0, true, tbl);
-      sc1[b] = combine_children<S, CN>(c1[b], ini1, true, tbl);
-      const T e0 = (em == EMITRIGHT) ? er0 : pick_esc<S>(ev0, xl0[b]);
-      const T e1 = (em == EMITRIGHT) ? er1 : pick_esc<S>(ev1, xl1[b]);
-      if (em != EMITNONE) { sc0[b] = R::add(sc0[b], e0); sc1[b] = R::add(sc1[b], e1); }
+      for (int k = 0; k < CN; k++) if (!(IR && t > 0 && k == 0)) c[t][b][k] = R::add(c[t][b][k], R::bits(s.tsc[k]));
+      const T ini = R::init(init_ok, el, s.endsc, dy[t][b]);
+      if (BEGL) {
+        sc[t][b] = ini;
+#pragma unroll
+        for (int k = 0; k < CN; k++) sc[t][b] = R::op(sc[t][b], c[t][b][k], tbl);
+      } else {
+        if (IR && t > 0) {
+          c[t][b][0] = R::add(R::fromb(shr1(R::tob(irc[t]), R::tob(sc[t - 1][b]))), R::bits(s.tsc[0]));
+          irc[t] = R::fromb(__builtin_amdgcn_readlane(R::tob(sc[t - 1][b]), 63));
+        }
+        sc[t][b] = combine_children<S, CN>(c[t][b], ini, true, tbl);
+        if (em != EMITNONE) sc[t][b] = R::add(sc[t][b], (em == EMITRIGHT) ? er[t] : pick_esc<S>(ev[t], xl[t][b]));
+      }
     }
   }
 #pragma unroll
-  for (int b = 0; b < NB; b++) {
-    if (a0[b]) {
-      stu(out0, (uint32_t) (s.base + d[b]), sc0[b]);
-      begin_update<S>(acc0, rdn, hb0 && d[b] >= bdn && d[b] <= bdx0, d[b], sc0[b], bsc);
-      if (VSC && vs0) vmax = fmaxf(vmax, R::score(sc0[b]));
-    }
-    if (a1[b]) {
-      stu(out1, (uint32_t) (s.base + d[b]), sc1[b]);
-      begin_update<S>(acc1, rdn, hb1 && d[b] >= bdn && d[b] <= bdx1, d[b], sc1[b], bsc);
-      if (VSC && vs1) vmax = fmaxf(vmax, R::score(sc1[b]));
-    }
-  }
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int t = 0; t < NR; t++)
+      if (a[t][b]) {
+        stu(obase, (uint32_t) (ooff[t] + s.base + d[b]), sc[t][b]);
+        begin_update<S>(acc0 + t * acc_n, rdn, hb && d[b] >= bdn && d[b] <= bdx[t], d[b], sc[t][b], bsc);
+        if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(sc[t][b]));
+      }
   return vmax;
 }
 
-/* A non-B, non-IL state with CN children over rows j (0) and j+1 (1), bands
- * [dn, dx0] and [dn, dx1] (dx1 < dn: no row 1).  BEGL decks combine init, c0,
- * c1, ... in child order (src/cm_dpsearch.c:430-444); other states use
- * combine_children.  NBW > 1 takes up to NBW blocks at a time where the band
- * has them (more loads in flight per wavefront, more registers): Inside with
- * the LDS table, whose log-sum chains wait on LDS and which runs at 4 waves
- * per SIMD anyway, and the wide-band kernels of models whose targets cannot
- * fill the chip with wavefronts.  Returns the max score over the rows
- * flagged in vs. */
-template <int S, bool VSC, int CN, bool BEGL, int NBW, bool IR = false>
-__device__ __forceinline__ float gen_state2(SW_PARAMS, const DState &s,
-                                            const typename SR<S>::T *crow0, const typename SR<S>::T *crow1,
-                                            typename SR<S>::T *out0, typename SR<S>::T *out1,
-                                            int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                            int ev0, int ev1, bool vs0, bool vs1)
+/* A non-B, non-IL state with CN children over rows j+t, t < NR (bands
+ * [dn, dx[t]]).  BEGL decks combine init, c0, c1, ... in child order
+ * (src/cm_dpsearch.c:430-444); other states use combine_children.  NBW > 1
+ * takes up to NBW blocks at a time where the band has them (more loads in
+ * flight per wavefront, more registers): the wide-band CYK kernel of models
+ * whose targets cannot fill the chip with wavefronts.  Returns the max score
+ * over the rows flagged in vs. */
+template <int S, bool VSC, int CN, bool BEGL, int NBW, bool IR, int NR>
+__device__ __forceinline__ float gen_state(SW_PARAMS, const DState &s,
+                                           const typename SR<S>::T *ibase, const int *ioff,
+                                           typename SR<S>::T *obase, const int *ooff,
+                                           int dn, const int *dx, bool hb, const int *bdx, const int *ev, const bool *vs)
 {
   typedef SR<S> R;
   typedef typename R::T T;
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
-  const int dxm = max(dx0, dx1);
+  int dxm = dx[0];
+#pragma unroll
+  for (int t = 1; t < NR; t++) dxm = max(dxm, dx[t]);
   float vmax = IMPF;
-  /* IR: row j's cell dn-1 (outside the band, never written by a visit) */
-  T irc = IR ? ldu(crow1, (uint32_t) (cb[0] + dn - 1)) : R::imp();
+  /* IR: row j+t-1's cell dn-1 (outside the band, never written by a visit) */
+  T irc[NR];
+#pragma unroll
+  for (int t = 0; t < NR; t++) irc[t] = (IR && t > 0) ? ldu(ibase, (uint32_t) (ioff[t] + cb[0] + dn - 1)) : R::imp();
   int d0 = dn;
   if (NBW >= 4)
     for (; d0 + 192 <= dxm; d0 += 256)
-      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 4, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                             hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 4, IR, NR>(SW_ARGS, s, ibase, ioff, obase, ooff, cb, d0, dn, dx, hb, bdx, ev, vs, irc));
   if (NBW >= 2)
     for (; d0 + 64 <= dxm; d0 += 128)
-      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                             hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
+      vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 2, IR, NR>(SW_ARGS, s, ibase, ioff, obase, ooff, cb, d0, dn, dx, hb, bdx, ev, vs, irc));
   for (; d0 <= dxm; d0 += 64)
-    vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1, IR>(SW_ARGS, s, crow0, crow1, out0, out1, cb, d0, dn, dx0, dx1,
-                                                           hb0, hb1, bdx0, bdx1, ev0, ev1, vs0, vs1, irc));
+    vmax = fmaxf(vmax, gen_blocks<S, VSC, CN, BEGL, 1, IR, NR>(SW_ARGS, s, ibase, ioff, obase, ooff, cb, d0, dn, dx, hb, bdx, ev, vs, irc));
   return vmax;
 }
 
@@ -456,11 +456,11 @@ __device__ __forceinline__ float il_guess_finside(float pre, float e, float t0, 
 
 /* IL state: x_d = op(pre_d, x_{d-1} + t0) + e_d with pre_d the other CN-1
  * child terms combined with init (the self term is applied last), solved
- * per 64-cell block by the exact fixed-point sweep, both rows together. */
-template <int S, bool VSC, int CN>
-__device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename SR<S>::T *cur0, typename SR<S>::T *cur1,
-                                           int dn, int dx0, int dx1, bool hb0, bool hb1, int bdx0, int bdx1,
-                                           int ev, bool vs0, bool vs1)
+ * per 64-cell block by the exact fixed-point sweep, the NR rows together
+ * (children on the same row: row j+t at rows[roff[t] + ...]). */
+template <int S, bool VSC, int CN, int NR>
+__device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename SR<S>::T *rows, const int *roff,
+                                          int dn, const int *dx, bool hb, const int *bdx, int ev, const bool *vs)
 {
   typedef SR<S> R;
   typedef typename R::T T;
@@ -471,68 +471,69 @@ __device__ __forceinline__ float il_state2(SW_PARAMS, const DState &s, typename 
   int cb[CN];
 #pragma unroll
   for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
-  const uint8_t *dsq_b1 = dsq_b0 + 1;
-  T carry0 = ldu(cur0, (uint32_t) (s.base + dn - 1));
-  T carry1 = ldu(cur1, (uint32_t) (s.base + dn - 1));
-  const int dxm = max(dx0, dx1);
+  T carry[NR];
+  int dxm = dx[0];
+#pragma unroll
+  for (int t = 0; t < NR; t++) {
+    carry[t] = ldu(rows, (uint32_t) (roff[t] + s.base + dn - 1));
+    dxm = max(dxm, dx[t]);
+  }
   float vmax = IMPF;
   for (int d0 = dn; d0 <= dxm; d0 += 64) {
     const int d = d0 + lane;
-    const bool a0 = d <= dx0, a1 = d <= dx1;
-    const int dd0 = a0 ? d : max(dn, dx0), dd1 = a1 ? d : max(dn, dx1);   /* idle lanes mirror a valid cell */
-    int xl0 = (int) ldu(dsq_b0, (uint32_t) (dsq_w - (a0 ? d : 1)));   /* idle lanes: residue j */
-    int xl1 = (int) ldu(dsq_b1, (uint32_t) (dsq_w - (a1 ? d : 1)));
-    T c0[CN], c1[CN];
-    c0[0] = R::imp(); c1[0] = R::imp();
+    bool a[NR];
+    int dd[NR], xl[NR];
+    T c[NR][CN];
 #pragma unroll
-    for (int k = 1; k < CN; k++) {
-      c0[k] = ldu(cur0, (uint32_t) (cb[k] + dd0 - 1));
-      c1[k] = ldu(cur1, (uint32_t) (cb[k] + dd1 - 1));
+    for (int t = 0; t < NR; t++) {
+      a[t] = d <= dx[t];
+      dd[t] = a[t] ? d : max(dn, dx[t]);   /* idle lanes mirror a valid cell */
+      xl[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t] ? d : 1)));   /* idle lanes: residue j+t */
+      c[t][0] = R::imp();
+#pragma unroll
+      for (int k = 1; k < CN; k++) c[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + dd[t] - 1));
     }
-    hold(xl0); hold(xl1);
 #pragma unroll
-    for (int k = 1; k < CN; k++) { hold(c0[k]); hold(c1[k]); }
+    for (int t = 0; t < NR; t++) {
+      hold(xl[t]);
 #pragma unroll
-    for (int k = 1; k < CN; k++) { c0[k] = R::add(c0[k], R::bits(s.tsc[k])); c1[k] = R::add(c1[k], R::bits(s.tsc[k])); }
-    const T pre0 = combine_children<S, CN>(c0, R::init(init_ok, el, s.endsc, dd0 - 1), false, tbl);
-    const T pre1 = combine_children<S, CN>(c1, R::init(init_ok, el, s.endsc, dd1 - 1), false, tbl);
-    const T e0 = pick_esc<S>(ev, xl0);
-    const T e1 = pick_esc<S>(ev, xl1);
-    const int cb0 = R::tob(carry0), cb1 = R::tob(carry1);
-    /* the start is only a guess: the sweeps run until every lane is a fixed
-     * point of the exact update, which the sequential recursion defines
-     * uniquely (lane k is exact after k + 1 sweeps from any start).  CYK and
-     * IInside start from pre + e (the first sweep brings in the carry);
-     * FInside from the prefix-scan estimate of the whole chain. */
-    T x0, x1;
-    if ((S & 7) == SEMI_FINSIDE) {
-      x0 = il_guess_finside(pre0, e0, t0, carry0);
-      x1 = il_guess_finside(pre1, e1, t0, carry1);
-    } else {
-      x0 = R::add(pre0, e0);
-      x1 = R::add(pre1, e1);
+      for (int k = 1; k < CN; k++) hold(c[t][k]);
+    }
+    T pre[NR], e[NR], x[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+#pragma unroll
+      for (int k = 1; k < CN; k++) c[t][k] = R::add(c[t][k], R::bits(s.tsc[k]));
+      pre[t] = combine_children<S, CN>(c[t], R::init(init_ok, el, s.endsc, dd[t] - 1), false, tbl);
+      e[t] = pick_esc<S>(ev, xl[t]);
+      /* the start is only a guess: the sweeps run until every lane is a
+       * fixed point of the exact update, which the sequential recursion
+       * defines uniquely (lane k is exact after k + 1 sweeps from any
+       * start).  CYK and IInside start from pre + e (the first sweep brings
+       * in the carry); FInside from the prefix-scan estimate of the chain. */
+      if ((S & 7) == SEMI_FINSIDE) x[t] = il_guess_finside(pre[t], e[t], t0, carry[t]);
+      else                         x[t] = R::add(pre[t], e[t]);
     }
     for (int it = 0; it < 66; it++) {
-      const T l0 = R::fromb(shr1(cb0, R::tob(x0)));
-      const T l1 = R::fromb(shr1(cb1, R::tob(x1)));
-      const T n0 = R::add(R::op(pre0, R::add(l0, t0), tbl), e0);
-      const T n1 = R::add(R::op(pre1, R::add(l1, t0), tbl), e1);
-      const bool changed = (a0 && !R::same(n0, x0)) || (a1 && !R::same(n1, x1));
-      x0 = n0; x1 = n1;
+      bool changed = false;
+#pragma unroll
+      for (int t = 0; t < NR; t++) {
+        const T l = R::fromb(shr1(R::tob(carry[t]), R::tob(x[t])));
+        const T n = R::add(R::op(pre[t], R::add(l, t0), tbl), e[t]);
+        changed = changed || (a[t] && !R::same(n, x[t]));
+        x[t] = n;
+      }
       if (!__any(changed)) break;
     }
-    if (a0) {
-      stu(cur0, (uint32_t) (s.base + d), x0);
-      begin_update<S>(acc0, rdn, hb0 && d >= bdn && d <= bdx0, d, x0, bsc);
-      if (VSC && vs0) vmax = fmaxf(vmax, R::score(x0));
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      if (a[t]) {
+        stu(rows, (uint32_t) (roff[t] + s.base + d), x[t]);
+        begin_update<S>(acc0 + t * acc_n, rdn, hb && d >= bdn && d <= bdx[t], d, x[t], bsc);
+        if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(x[t]));
+      }
+      carry[t] = R::fromb(__builtin_amdgcn_readlane(R::tob(x[t]), 63));
     }
-    if (a1) {
-      stu(cur1, (uint32_t) (s.base + d), x1);
-      begin_update<S>(acc1, rdn, hb1 && d >= bdn && d <= bdx1, d, x1, bsc);
-      if (VSC && vs1) vmax = fmaxf(vmax, R::score(x1));
-    }
-    carry0 = R::fromb(__builtin_amdgcn_readlane(R::tob(x0), 63));
-    carry1 = R::fromb(__builtin_amdgcn_readlane(R::tob(x1), 63));
   }
   return vmax;
 }
@@ -647,11 +648,13 @@ __device__ __forceinline__ int begin_argmax(const DState *stp, int nst, const ty
   return br;
 }
 
-template <int S, bool VSC, bool TBL_LDS, int NBC = 1, bool BATCH = false>
+template <int S, bool VSC, bool TBL_LDS, int NBC = 1, bool BATCH = false, int NR = 2>
 __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 {
-  /* blocks per step of the generic states (gen_state2) */
-  constexpr int NBW = ((S & 7) != SEMI_CYK && TBL_LDS) ? (NBC > 2 ? NBC : 2) : NBC;
+  /* blocks per step of the generic states (gen_state); with the LDS table
+   * Inside keeps 2 blocks of 2 rows, or 1 block of 4 rows, in flight */
+  constexpr int NBW = ((S & 7) != SEMI_CYK && TBL_LDS && NR == 2) ? (NBC > 2 ? NBC : 2) : NBC;
+  constexpr int RM = 2 * NR - 1;   /* row ring mask */
   typedef SR<S> R;
   typedef typename R::T T;
   extern __shared__ __align__(16) char lds_raw[];
@@ -676,9 +679,8 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 
   const int acc_n = fresh(ap)->acc_n;
   char *sbase = fresh(ap)->scratch + (size_t) slot * fresh(ap)->slot_bytes;
-  /* local-begin maxima of rows j, j+1 (global: int keys) */
-  T *acc0 = R::GACC ? (T *) (sbase + fresh(ap)->acc_off) : (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 8);
-  T *acc1 = acc0 + acc_n;
+  /* local-begin maxima of rows j .. j+NR-1 (global: int keys) */
+  T *acc0 = R::GACC ? (T *) (sbase + fresh(ap)->acc_off) : (T *) (lds_raw + lds_off + (size_t) wib * acc_n * 4 * NR);
 
   T   *deck = (T *) (sbase + fresh(ap)->deck_off);
   T   *rows = (T *) (sbase + fresh(ap)->row_off);
@@ -711,7 +713,11 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 
     {
       const T *rtmpl = (const T *) fresh(cp)->row_tmpl;
-      for (int x = lane; x < R; x += 64) { T t = rtmpl[x]; rows[x] = t; rows[R + x] = t; rows[2 * R + x] = t; rows[3 * R + x] = t; }
+      for (int x = lane; x < R; x += 64) {
+        const T t = rtmpl[x];
+#pragma unroll
+        for (int r = 0; r <= RM; r++) rows[r * R + x] = t;
+      }
       const T *dtmpl = (const T *) fresh(cp)->deck_tmpl;
       const int RB = fresh(cp)->RB;
       for (int x = lane; x < RB; x += 64) deck[x] = dtmpl[x];
@@ -721,31 +727,34 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
     float   best = IMPF;
     int64_t envi = INT64_MAX, envj = INT64_MIN;
 
-    for (int64_t j0 = s0; j0 <= bc; j0 += 2) {
-      const bool two = j0 + 1 <= bc;                 /* row j0+1 is in this chunk */
-      const int jp0 = (int) (j0 - s0 + 1), jp1 = jp0 + 1;
-      T *cur0 = rows + (size_t) ((j0 + 0) & 3) * R;
-      T *cur1 = rows + (size_t) ((j0 + 1) & 3) * R;
-      T *prv0 = rows + (size_t) ((j0 + 3) & 3) * R;   /* row j0-1 */
+    for (int64_t j0 = s0; j0 <= bc; j0 += NR) {
+      const int nr = (int) min((int64_t) NR, bc - j0 + 1);   /* rows of this pass inside the chunk */
+      const int jp0 = (int) (j0 - s0 + 1);
       const uint8_t *dsq_j = fresh(cp)->dsq + (j0 + 1 - fresh(cp)->dsq_off);
-      const int xj0 = __builtin_amdgcn_readfirstlane((int) dsq_j[-1]);
-      const int xj1 = two ? __builtin_amdgcn_readfirstlane((int) dsq_j[0]) : 0;
       const int rdn = fresh(cp)->root_dn;
-      const int rdx0 = min(jp0, fresh(cp)->root_dx);
-      const int rdx1 = two ? min(jp1, fresh(cp)->root_dx) : rdn - 1;
-      const bool vs0 = j0 >= ac, vs1 = two && j0 + 1 >= ac;
+      int roff[NR], xj[NR], rdx[NR];
+      bool vs[NR];
+#pragma unroll
+      for (int t = 0; t < NR; t++) {
+        roff[t] = (int) ((j0 + t) & RM) * R;                 /* row j0+t */
+        xj[t] = t < nr ? __builtin_amdgcn_readfirstlane((int) dsq_j[t - 1]) : 0;
+        rdx[t] = t < nr ? min(jp0 + t, fresh(cp)->root_dx) : rdn - 1;
+        vs[t] = t < nr && j0 + t >= ac;
+      }
+      const int poff = (int) ((j0 + RM) & RM) * R;           /* row j0-1 */
       const int local_begin = fresh(cp)->local_begin;
       const uint32_t el = fresh(cp)->el_selfsc;
       const DState *stp = fresh(cp)->st;
       const T *esc = (const T *) fresh(cp)->esc;
       const int nst = fresh(cp)->nst;
       const int dsq_w = fresh(cp)->W + 64;
-      const uint8_t *dsq_b0 = dsq_j - dsq_w;   /* left residue of cell d of row j0: dsq_b0[dsq_w - d] */
+      const uint8_t *dsq_b0 = dsq_j - dsq_w;   /* left residue of cell d of row j0+t: dsq_b0[t + dsq_w - d] */
 
       if (local_begin) {
         const T neg = R::GACC ? R::fromb(R::key(R::neg_sentinel())) : R::neg_sentinel();
-        for (int d = rdn + lane; d <= rdx0; d += 64) acc0[d - rdn] = neg;
-        for (int d = rdn + lane; d <= rdx1; d += 64) acc1[d - rdn] = neg;
+#pragma unroll
+        for (int t = 0; t < NR; t++)
+          for (int d = rdn + lane; d <= rdx[t]; d += 64) acc0[t * acc_n + d - rdn] = neg;
         if (R::GACC) __builtin_amdgcn_s_waitcnt(0);   /* the clears reach L2 before any atomic */
       }
 
@@ -755,37 +764,45 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
         const int kind = hdr & 7;
         const int dn = s.dnx & 0xffff;
         const int sdx = (int) (s.dnx >> 16);
-        const int dx0 = min(jp0, sdx);
-        const int dx1 = two ? min(jp1, sdx) : dn - 1;
-        if (dn > max(dx0, dx1)) continue;
+        int dx[NR], bdx[NR];
+        int dxm = dn - 1;
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+          dx[t] = t < nr ? min(jp0 + t, sdx) : dn - 1;
+          bdx[t] = min(rdx[t], dx[t]);
+          dxm = max(dxm, dx[t]);
+        }
+        if (dn > dxm) continue;
         const int v = hdr >> 16;
         const bool hb = local_begin && ((hdr >> 11) & 1);
         const int cnum = (hdr >> 8) & 7;
         const int em = (hdr >> 3) & 3;
-        const int bdx0 = min(rdx0, dx0), bdx1 = min(rdx1, dx1);
         /* emission rows: lane y <- score of residue code y (MP: left code y, right code xj) */
         /* Loaded for every state (non-emitting states read entry 0 and never
          * use it): a branch here would join a loaded and a non-loaded value,
          * and the register write on the other path has to wait for any load
-         * still in flight -- the previous state's stores included.  Two
-         * independent loads (the same row twice for singlets), so no value is
-         * needed before the state's child loads are issued. */
+         * still in flight -- the previous state's stores included.  One
+         * independent load per row (the same row for singlets), so no value
+         * is needed before the state's child loads are issued. */
         const int yy = lane < KP ? lane : 0;
-        const uint32_t eo0 = (em == EMITNONE) ? 0u : (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj0 : yy));
-        const uint32_t eo1 = (em == EMITNONE) ? 0u : (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj1 : yy));
-        const int ev0 = R::tob(ldu(esc, eo0));
-        const int ev1 = R::tob(ldu(esc, eo1));
+        int ev[NR];
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+          const uint32_t eo = (em == EMITNONE) ? 0u : (uint32_t) (s.aux + (em == EMITPAIR ? yy * KP + xj[t] : yy));
+          ev[t] = R::tob(ldu(esc, eo));
+        }
         const bool rd = (hdr >> 7) & 1;
-        const T *crow0 = rd ? prv0 : cur0;
-        const T *crow1 = rd ? cur0 : cur1;
+        int ioff[NR];   /* children's rows: j0+t, or j0+t-1 for right emitters */
+#pragma unroll
+        for (int t = 0; t < NR; t++) ioff[t] = rd ? (t == 0 ? poff : roff[t - 1]) : roff[t];
         float vmax;
         if (kind == KIND_GENERIC) {
-          /* IR reads its own row j cells for row j+1: gen_blocks<IR> takes
-           * them from row j's results in registers */
+          /* IR reads its own row j0+t-1 cells for row j0+t: gen_blocks<IR>
+           * takes them from the previous row's results in registers */
           const bool seq = (hdr >> 14) & 1;
           if (seq) {
             switch (cnum) {
-#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW, true>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vmax = gen_state<S, VSC, CN, false, NBW, true, NR>(SW_ARGS, s, rows, ioff, rows, roff, dn, dx, hb, bdx, ev, vs)
             case 2:  GEN(2); break;
             case 3:  GEN(3); break;
             case 4:  GEN(4); break;
@@ -795,7 +812,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
             }
           } else {
             switch (cnum) {
-#define GEN(CN) vmax = gen_state2<S, VSC, CN, false, NBW>(SW_ARGS, s, crow0, crow1, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, ev1, vs0, vs1)
+#define GEN(CN) vmax = gen_state<S, VSC, CN, false, NBW, false, NR>(SW_ARGS, s, rows, ioff, rows, roff, dn, dx, hb, bdx, ev, vs)
             case 2:  GEN(2); break;
             case 3:  GEN(3); break;
             case 4:  GEN(4); break;
@@ -806,7 +823,7 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           }
         } else if (kind == KIND_IL) {
           switch (cnum) {
-#define ILS(CN) vmax = il_state2<S, VSC, CN>(SW_ARGS, s, cur0, cur1, dn, dx0, dx1, hb, hb, bdx0, bdx1, ev0, vs0, vs1)
+#define ILS(CN) vmax = il_state<S, VSC, CN, NR>(SW_ARGS, s, rows, roff, dn, dx, hb, bdx, ev[0], vs)
           case 2:  ILS(2); break;
           case 3:  ILS(3); break;
           case 4:  ILS(4); break;
@@ -816,10 +833,11 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           }
         } else if (kind == KIND_BEGL) {
           const int rmask = s.aux & 0xffff, rstride = (int) (s.aux >> 16);
-          T *out0 = deck + (jp0 & rmask) * rstride;
-          T *out1 = deck + (jp1 & rmask) * rstride;
+          int ooff[NR];
+#pragma unroll
+          for (int t = 0; t < NR; t++) ooff[t] = ((jp0 + t) & rmask) * rstride;
           switch (cnum) {
-#define BGL(CN) vmax = gen_state2<S, VSC, CN, true, NBW>(SW_ARGS, s, crow0, crow1, out0, out1, dn, dx0, dx1, false, false, bdx0, bdx1, 0, 0, vs0, vs1)
+#define BGL(CN) vmax = gen_state<S, VSC, CN, true, NBW, false, NR>(SW_ARGS, s, rows, ioff, deck, ooff, dn, dx, false, bdx, ev, vs)
           case 1:  BGL(1); break;
           case 2:  BGL(2); break;
           case 3:  BGL(3); break;
@@ -829,34 +847,43 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
 #undef BGL
           }
         } else {
-          vmax = b_state<S, VSC>(tbl, el, lane, acc0, rdn, s, cur0, deck, jp0, dn, dx0, hb, bdx0, vs0);
-          wave_sync();
-          if (two) vmax = fmaxf(vmax, b_state<S, VSC>(tbl, el, lane, acc1, rdn, s, cur1, deck, jp1, dn, dx1, hb, bdx1, vs1));
+          vmax = IMPF;
+#pragma unroll
+          for (int t = 0; t < NR; t++) {
+            if (t >= nr) break;
+            if (t > 0) wave_sync();
+            vmax = fmaxf(vmax, b_state<S, VSC>(tbl, el, lane, acc0 + t * acc_n, rdn, s, rows + roff[t], deck, jp0 + t, dn, dx[t], hb, bdx[t], vs[t]));
+          }
         }
         if (VSC) {
           for (int o = 32; o >= 1; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-          if (lane == 0 && (vs0 || vs1)) vscs[v] = fmaxf(vscs[v], vmax);
+          bool any_vs = false;
+#pragma unroll
+          for (int t = 0; t < NR; t++) any_vs = any_vs || vs[t];
+          if (lane == 0 && any_vs) vscs[v] = fmaxf(vscs[v], vmax);
         }
         wave_sync();
       } /* states */
 
       if (R::GACC && local_begin) __builtin_amdgcn_s_waitcnt(0);   /* every atomic done before the root reads */
       /* ROOT_S (src/cm_dpsearch.c:562-623) and hit candidates, row by row */
-      for (int t = 0; t < (two ? 2 : 1); t++) {
+#pragma unroll
+      for (int t = 0; t < NR; t++) {
+        if (t >= nr) break;
         const int64_t j = j0 + t;
         if (j < ac) continue;
         const int jp = jp0 + t;
-        const T *cur = t ? cur1 : cur0;
-        const T *acc = t ? acc1 : acc0;
-        const int rdx = t ? rdx1 : rdx0;
+        const T *cur = rows + roff[t];
+        const T *acc = acc0 + t * acc_n;
+        const int rdxt = rdx[t];
         CArgs *aq = fresh(cp);
         const int root_cnum = aq->root_cnum;
         const int cand_mode = aq->cand_mode;
         const float cutoff = aq->cutoff, env_cutoff = aq->env_cutoff;
         float carry_max = -INFINITY;   /* prefix max of reportable scores (record filter) */
-        for (int d0 = rdn; d0 <= rdx; d0 += 64) {
+        for (int d0 = rdn; d0 <= rdxt; d0 += 64) {
           const int d = d0 + lane;
-          const bool act = d <= rdx;
+          const bool act = d <= rdxt;
           float bs = IMPF;
           bool lb = false;
           if (act) {
@@ -936,9 +963,14 @@ template <bool VSC, int G = 0>
 __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(4))) std_cyk_wide_kernel(const ScanArgs *a)
 { std_scan_body<SEMI_CYK | G, VSC, false, 4>(a); }
 
+/* rows per state visit of Inside with the LDS table (one 16-wavefront
+ * workgroup per CU: registers for four rows) */
+#define INFX_NR_INSIDE_LDS 4
+static_assert(INFX_NR_INSIDE_LDS <= INFX_NR_MAX, "BEGL deck rings allow for INFX_NR_MAX rows per visit");
+
 template <int S, bool VSC>
 __global__ void __launch_bounds__(64 * WPB_MAX) std_inside_lds_kernel(const ScanArgs *a)
-{ std_scan_body<S, VSC, true>(a); }
+{ std_scan_body<S, VSC, true, 1, false, INFX_NR_INSIDE_LDS>(a); }
 
 /* batched launches: chunks of several models' windows in one launch
  * (chunk table rows of four: first row, last row, window start, model) */
@@ -982,11 +1014,19 @@ static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds, int gacc,
   return NULL;
 }
 
+/* rows per state visit (NR) of a launch: the slot holds a ring of 2*NR rows
+ * and NR rows of local-begin maxima per wavefront (batched launches: 2) */
+extern "C" int infx_std_rows(int semi, int tbl_lds, int wide)
+{
+  return (semi != SEMI_CYK && tbl_lds && !wide) ? INFX_NR_INSIDE_LDS : 2;
+}
+
 /* LDS bytes of a workgroup of `wpb` wavefronts (gacc: accumulators in the
  * global slot, LDS holds only the Inside log-sum table) */
 extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc)
 {
-  return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) + (gacc ? 0 : (size_t) wpb * acc_n * 8);   /* two rows of 4-byte maxima */
+  return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) +
+         (gacc ? 0 : (size_t) wpb * acc_n * 4 * infx_std_rows(semi, tbl_lds, 0));   /* NR rows of 4-byte maxima */
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
@@ -1024,11 +1064,17 @@ static const void *std_batch_kernel_ptr(int semi)
   return NULL;
 }
 
+/* LDS bytes of a batched workgroup (two rows per state visit) */
+extern "C" size_t infx_std_batch_lds_bytes(int semi, int wpb, int acc_n)
+{
+  return (semi != SEMI_CYK ? (size_t) INFX_LOGSUM_PAD * 4 : 0) + (size_t) wpb * acc_n * 4 * 2;
+}
+
 extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n)
 {
   const void *f = std_batch_kernel_ptr(semi);
   if (!f || wpb < 1 || wpb > WPB_MAX) return 0;
-  const size_t lds = infx_std_lds_bytes(semi, 1, wpb, acc_n, 0);
+  const size_t lds = infx_std_batch_lds_bytes(semi, wpb, acc_n);
   if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
@@ -1039,7 +1085,7 @@ extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hip
 {
   const void *f = std_batch_kernel_ptr(semi);
   if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
-  const size_t lds = infx_std_lds_bytes(semi, 1, block.x / 64, acc_n, 0);
+  const size_t lds = infx_std_batch_lds_bytes(semi, block.x / 64, acc_n);
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
   void *args[] = { (void *) &a };

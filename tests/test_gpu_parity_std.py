@@ -233,3 +233,18 @@ def test_pinned_target():
     assert hits_equal(sorted(g["hits"]), sorted(u["hits"]))
     assert np.float32(g["sc"]).view(np.uint32) == np.float32(u["sc"]).view(np.uint32)
     _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, 1, dsq, 2301, 6000, 0.0, env=3.0)
+
+
+@pytest.mark.parametrize("name,qdb,L", [("rnaseP-eubact.c.cm", 1, 5000), ("Plant_SRP.c.cm", 1, 5000),
+                                        ("synth_nested1500.cm", 2, 1800)])
+def test_bifurcation_split_at_deck_ring_edge(name, qdb, L):
+    """Band sets in which a bifurcation's largest split k is 63.  A state
+    visit writes the BEGL deck rows jp .. jp+NR-1 before the bifurcation
+    reads rows jp+t-k down to jp-63, so the deck ring must hold 63 + NR rows;
+    a ring of 64 would hand the k = 63 term a row written in the same visit.
+    CYK and FInside against the oracle, per-state maxima included."""
+    cm = load_cm(name)
+    dsq = random_dsq(L, seed=31)
+    if qdb == 1:
+        _run(ia.FastCYKScan, ob.ORACLE_CYK, cm, qdb, dsq, 1, L, 0.0, env=3.0, vsc=True)
+    _run(ia.FastFInsideScan, ob.ORACLE_FINSIDE, cm, qdb, dsq, 1, L, 0.0, env=3.0, vsc=True)

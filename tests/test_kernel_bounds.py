@@ -2,7 +2,7 @@
 
 The scan profile the library uploads (state descriptors, row/deck sizes,
 emission table, root band) is taken from infx_debug_std_profile, and the
-index formulas of infernal_amd/csrc/scan_std.hip (two-row sweeps, 64-lane
+index formulas of infernal_amd/csrc/scan_std.hip (multi-row sweeps, 64-lane
 blocks, idle-lane stand-in cells, the bifurcation k-loop with its 4-wide
 overshoot, local-begin accumulators, root, emission rows, residue reads) are
 evaluated for every state over the chunk's warm-up and steady rows and at the
@@ -11,7 +11,7 @@ target's end.  Every access must fall inside its buffer:
   residues  device bytes [64, L + 128): the target itself and the 64 zero
             bytes after it (the 64 bytes before it are a guard the kernel
             must not rely on)
-  rows      [0, R) of each of the four ring rows
+  rows      [0, R) of each ring row (the kernel adds the ring row's offset)
   decks     [-G, RB + G) with G = W + 128 rounded up to 64 (guard cells)
   LDS       accumulator index [0, acc_n)
   emission  [0, nesc)
@@ -21,7 +21,10 @@ that take two or four blocks per step (Inside with the LDS table, the
 wide-band CYK kernel) visit the same blocks with the same per-block masks,
 which test_multi_block_steps_visit_the_same_blocks checks; the kernels with
 the local-begin maxima in the global slot use the same accumulator indices
-(row j at [0, acc_n), row j+1 at [acc_n, 2 acc_n) of the slot's region).
+(row j+t at [t acc_n, (t+1) acc_n) of the slot's region).  A visit of NR
+rows accesses, for each of its rows, what the two-row replay below checks for
+a row at the same chunk position (the rows of one visit differ only in jp),
+plus the residue j0+t of each idle row past the chunk's end.
 
 This runs without a GPU, before any kernel change goes to one.
 """
@@ -35,6 +38,7 @@ import infernal_amd as ia
 
 CMDIR = os.path.join(os.path.dirname(__file__), "golden", "cm")
 KP = 18
+NR_MAX = 4          # most rows a kernel computes per state visit (scan_kernels.h INFX_NR_MAX)
 KIND_GENERIC, KIND_IL, KIND_B, KIND_BEGL = 0, 1, 2, 3
 EMITLEFT, EMITRIGHT, EMITPAIR, EMITNONE = 0, 1, 2, 3
 LANES = np.arange(64)
@@ -246,6 +250,9 @@ def run_checks(name, local, qdbidx, L, semi=0):
             rdx0 = min(jp0, root_dx)
             rdx1 = min(jp0 + 1, root_dx) if two else rdn - 1
             check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, jp0, local)
+    # a visit of NR rows reads the residue j0+t of each idle row t (past the
+    # chunk's end: up to j0+NR-1 with j0 <= L), on every lane
+    chk.dsq(np.array([64 + L + NR_MAX - 2]), "idle-row residue at the target's end")
     # the target's end (last chunk, steady-state bands): a row pair ending at L and a lone row at L
     for j0, two in ((L - 1, True), (L, False)):
         if j0 < 1:
@@ -270,6 +277,37 @@ def test_std_kernel_addresses_in_bounds(name):
             lengths = [1, 2, 7, 64, W - 1, W, W + 1, 3 * W] if small else [W - 1, 3 * W]
             for L in lengths:
                 run_checks(name, local, qdbidx, L)
+
+
+@pytest.mark.parametrize("name", ["tRNA.c.cm", "Cobalamin.c.cm", "rnaseP-eubact.c.cm", "synth_multiloop350.cm",
+                                  "synth_nested1500.cm"])
+def test_begl_rings_hold_one_visit(name):
+    """A state visit writes rows jp .. jp+NR-1 of a BEGL deck before its B
+    parent reads deck rows jp+t-k, k up to the largest split maxk: the ring
+    (rmask + 1 rows) must keep rows jp-maxk .. jp+NR-1 apart, i.e. hold
+    maxk + NR rows, for every bifurcation, band set and mode."""
+    for local in (True, False):
+        cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=local)
+        for qdbidx in (0, 1, 2):
+            st = profile(cm, qdbidx, 0, 100000)[0]
+            nb = 0
+            for w in st:
+                hdr = int(w[0])
+                if hdr & 7 != KIND_B:
+                    continue
+                nb += 1
+                dn, sdx = int(w[1]) & 0xffff, int(w[1]) >> 16
+                kdn_y, kdx_y = int(w[6]) & 0xffff, int(w[6]) >> 16
+                kdn_w, kdx_w = int(w[7]) & 0xffff, int(w[7]) >> 16
+                if (hdr >> 13) & 1:
+                    maxk = max((min(kdx_y, d - kdn_w) for d in range(dn, sdx + 1)
+                                if max(0, kdn_y, d - kdx_w) <= min(kdx_y, d - kdn_w)), default=0)
+                else:
+                    maxk = sdx
+                ring = (int(w[3]) & 0xffff) + 1
+                assert ring & (ring - 1) == 0
+                assert ring >= maxk + NR_MAX, "%s qdb %d state %d: ring %d, maxk %d" % (name, qdbidx, hdr >> 16, ring, maxk)
+            assert nb > 0
 
 
 def test_checker_catches_planted_overreach():
