@@ -18,7 +18,7 @@
 
 extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int gacc, int wide, dim3 grid, dim3 block,
                                       hipStream_t stream, const ScanArgs *a, int acc_n);
-extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc);
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
 extern "C" size_t infx_std_batch_lds_bytes(int semi, int wpb, int acc_n);
 extern "C" int infx_std_rows(int semi, int tbl_lds, int wide);
 extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
@@ -691,7 +691,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
       int bw = 0, bo = 0;   /* the workgroup size that keeps the most wavefronts resident (ties: larger) */
       for (int w = 16; w >= 1; w >>= 1) {
         if (infx_wpb_override() && w != infx_wpb_override()) continue;
-        if (infx_std_lds_bytes(semi, tbl_lds, w, acc_n, g) > (size_t) g_lds_max) continue;
+        if (infx_std_lds_bytes(semi, tbl_lds, w, acc_n, g, wide) > (size_t) g_lds_max) continue;
         const int o = infx_std_occupancy(semi, ret_vsc != NULL, tbl_lds, w, acc_n, g, wide);
         if (o * w > bo * bw) { bw = w; bo = o; }
       }

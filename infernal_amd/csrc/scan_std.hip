@@ -346,11 +346,13 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
   T sc[NR][NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
+    /* the init term of cell d: the same in every row (idle lanes get their
+     * own d's, a value nothing reads) */
+    const T ini = R::init(init_ok, el, s.endsc, d[b] - sd);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
 #pragma unroll
       for (int k = 0; k < CN; k++) if (!(IR && t > 0 && k == 0)) c[t][b][k] = R::add(c[t][b][k], R::bits(s.tsc[k]));
-      const T ini = R::init(init_ok, el, s.endsc, dy[t][b]);
       if (BEGL) {
         sc[t][b] = ini;
 #pragma unroll
@@ -500,11 +502,12 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
       for (int k = 1; k < CN; k++) hold(c[t][k]);
     }
     T pre[NR], e[NR], x[NR];
+    const T ini = R::init(init_ok, el, s.endsc, d - 1);   /* every row's (idle lanes: a value nothing reads) */
 #pragma unroll
     for (int t = 0; t < NR; t++) {
 #pragma unroll
       for (int k = 1; k < CN; k++) c[t][k] = R::add(c[t][k], R::bits(s.tsc[k]));
-      pre[t] = combine_children<S, CN>(c[t], R::init(init_ok, el, s.endsc, dd[t] - 1), false, tbl);
+      pre[t] = combine_children<S, CN>(c[t], ini, false, tbl);
       e[t] = pick_esc<S>(ev, xl[t]);
       /* the start is only a guess: the sweeps run until every lane is a
        * fixed point of the exact update, which the sequential recursion
@@ -533,6 +536,120 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
         if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(x[t]));
       }
       carry[t] = R::fromb(__builtin_amdgcn_readlane(R::tob(x[t]), 63));
+    }
+  }
+  return vmax;
+}
+
+/* FInside IL states with bands wider than one block: 128-cell blocks, two
+ * consecutive cells per lane (d0 + 2l and d0 + 2l + 1).  A sweep takes the
+ * left neighbour's second cell (DPP wave_shr:1), computes the lane's first
+ * cell from it and its second from the first, so each sweep carries the
+ * chain two cells on; lane l is exact after l + 1 sweeps from any start, and
+ * a sweep that changes no lane is the unique fixed point, as in il_state.
+ * The starting point is il_guess_finside's prefix over the lanes, applied to
+ * the composition of each lane's two maps. */
+__device__ __forceinline__ void il_guess_finside2(float preA, float eA, float preB, float eB, float t0, float carry,
+                                                  float &xA, float &xB)
+{
+  const float a0 = __fadd_rn(t0, eA), b0 = __fadd_rn(preA, eA);
+  const float a1 = __fadd_rn(t0, eB), b1 = __fadd_rn(preB, eB);
+  float A = __fadd_rn(a1, a0), B = lse2_hw(b1, __fadd_rn(a1, b0));   /* second cell's map from the lane's input */
+  il_scan_step<0x111, 0xf>(A, B);   /* row_shr:1 */
+  il_scan_step<0x112, 0xf>(A, B);   /* row_shr:2 */
+  il_scan_step<0x114, 0xf>(A, B);   /* row_shr:4 */
+  il_scan_step<0x118, 0xf>(A, B);   /* row_shr:8 */
+  il_scan_step<0x142, 0xa>(A, B);   /* row_bcast:15 into rows 1 and 3 */
+  il_scan_step<0x143, 0xc>(A, B);   /* row_bcast:31 into rows 2 and 3 */
+  const float xlast = lse2_hw(B, __fadd_rn(A, carry));                /* the lane's second cell */
+  const float xin = __int_as_float(shr1(__float_as_int(carry), __float_as_int(xlast)));   /* its input */
+  xA = lse2_hw(b0, __fadd_rn(a0, xin));
+  xB = lse2_hw(b1, __fadd_rn(a1, xA));
+}
+
+template <int S, bool VSC, int CN, int NR>
+__device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typename SR<S>::T *rows, const int *roff,
+                                             int dn, const int *dx, bool hb, const int *bdx, int ev, const bool *vs)
+{
+  typedef SR<S> R;
+  typedef typename R::T T;
+  const int init_ok = (s.hdr >> 12) & 1;
+  const T bsc = R::bits(s.beginsc);
+  const T t0 = R::bits(s.tsc[0]);
+  const int bdn = max(rdn, dn);
+  int cb[CN];
+#pragma unroll
+  for (int k = 0; k < CN; k++) cb[k] = s.cbase + (int) (int16_t) (s.coff[k >> 1] >> (16 * (k & 1)));
+  T carry[NR];
+  int dxm = dx[0];
+#pragma unroll
+  for (int t = 0; t < NR; t++) {
+    carry[t] = ldu(rows, (uint32_t) (roff[t] + s.base + dn - 1));
+    dxm = max(dxm, dx[t]);
+  }
+  float vmax = IMPF;
+  for (int d0 = dn; d0 <= dxm; d0 += 128) {
+    const int dA = d0 + 2 * lane, dB = dA + 1;
+    bool aA[NR], aB[NR];
+    int xlA[NR], xlB[NR];
+    T cA[NR][CN], cB[NR][CN];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      aA[t] = dA <= dx[t]; aB[t] = dB <= dx[t];
+      const int sv = max(dn, dx[t]);                      /* idle lanes mirror a valid cell */
+      const int ddA = aA[t] ? dA : sv, ddB = aB[t] ? dB : sv;
+      xlA[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (aA[t] ? dA : 1)));   /* idle lanes: residue j+t */
+      xlB[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (aB[t] ? dB : 1)));
+      cA[t][0] = R::imp(); cB[t][0] = R::imp();
+#pragma unroll
+      for (int k = 1; k < CN; k++) {
+        cA[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + ddA - 1));
+        cB[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + ddB - 1));
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      hold(xlA[t]); hold(xlB[t]);
+#pragma unroll
+      for (int k = 1; k < CN; k++) { hold(cA[t][k]); hold(cB[t][k]); }
+    }
+    const T iniA = R::init(init_ok, el, s.endsc, dA - 1), iniB = R::init(init_ok, el, s.endsc, dB - 1);
+    T preA[NR], preB[NR], eA[NR], eB[NR], xA[NR], xB[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+#pragma unroll
+      for (int k = 1; k < CN; k++) { cA[t][k] = R::add(cA[t][k], R::bits(s.tsc[k])); cB[t][k] = R::add(cB[t][k], R::bits(s.tsc[k])); }
+      preA[t] = combine_children<S, CN>(cA[t], iniA, false, tbl);
+      preB[t] = combine_children<S, CN>(cB[t], iniB, false, tbl);
+      eA[t] = pick_esc<S>(ev, xlA[t]);
+      eB[t] = pick_esc<S>(ev, xlB[t]);
+      il_guess_finside2(preA[t], eA[t], preB[t], eB[t], t0, carry[t], xA[t], xB[t]);
+    }
+    for (int it = 0; it < 66; it++) {
+      bool changed = false;
+#pragma unroll
+      for (int t = 0; t < NR; t++) {
+        const T l = R::fromb(shr1(R::tob(carry[t]), R::tob(xB[t])));
+        const T nA = R::add(R::op(preA[t], R::add(l, t0), tbl), eA[t]);
+        const T nB = R::add(R::op(preB[t], R::add(nA, t0), tbl), eB[t]);
+        changed = changed || (aA[t] && !R::same(nA, xA[t])) || (aB[t] && !R::same(nB, xB[t]));
+        xA[t] = nA; xB[t] = nB;
+      }
+      if (!__any(changed)) break;
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      if (aA[t]) {
+        stu(rows, (uint32_t) (roff[t] + s.base + dA), xA[t]);
+        begin_update<S>(acc0 + t * acc_n, rdn, hb && dA >= bdn && dA <= bdx[t], dA, xA[t], bsc);
+        if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(xA[t]));
+      }
+      if (aB[t]) {
+        stu(rows, (uint32_t) (roff[t] + s.base + dB), xB[t]);
+        begin_update<S>(acc0 + t * acc_n, rdn, hb && dB >= bdn && dB <= bdx[t], dB, xB[t], bsc);
+        if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(xB[t]));
+      }
+      carry[t] = R::fromb(__builtin_amdgcn_readlane(R::tob(xB[t]), 63));
     }
   }
   return vmax;
@@ -823,7 +940,12 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
           }
         } else if (kind == KIND_IL) {
           switch (cnum) {
-#define ILS(CN) vmax = il_state<S, VSC, CN, NR>(SW_ARGS, s, rows, roff, dn, dx, hb, bdx, ev[0], vs)
+#define ILS(CN) if constexpr ((S & 7) == SEMI_FINSIDE) { \
+                  vmax = (dxm - dn >= 64) ? il_state_w2<S, VSC, CN, NR>(SW_ARGS, s, rows, roff, dn, dx, hb, bdx, ev[0], vs) \
+                                          : il_state<S, VSC, CN, NR>(SW_ARGS, s, rows, roff, dn, dx, hb, bdx, ev[0], vs); \
+                } else { \
+                  vmax = il_state<S, VSC, CN, NR>(SW_ARGS, s, rows, roff, dn, dx, hb, bdx, ev[0], vs); \
+                }
           case 2:  ILS(2); break;
           case 3:  ILS(3); break;
           case 4:  ILS(4); break;
@@ -950,12 +1072,15 @@ __device__ __forceinline__ void std_scan_body(const ScanArgs *ap_)
   } /* chunks */
 }
 
-/* Kernels.  CYK and Inside with the log-sum table read from L1/L2 are
- * register-allocated for 8 wavefronts per SIMD; Inside with the table in LDS
- * is limited to one 16-wavefront workgroup per CU by the table itself. */
+/* Kernels.  CYK visits each state for four rows (6 wavefronts per SIMD:
+ * 80 VGPRs; measured faster than two rows at 8 wavefronts); Inside
+ * with the log-sum table read from L1/L2 is register-allocated for 8
+ * wavefronts per SIMD; Inside with the table in LDS is limited to one
+ * 16-wavefront workgroup per CU by the table itself. */
+#define INFX_NR_CYK 4
 template <bool VSC, int G = 0>
-__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(8))) std_cyk_kernel(const ScanArgs *a)
-{ std_scan_body<SEMI_CYK | G, VSC, false>(a); }
+__global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_eu(6))) std_cyk_kernel(const ScanArgs *a)
+{ std_scan_body<SEMI_CYK | G, VSC, false, 1, false, INFX_NR_CYK>(a); }
 
 /* wide-band CYK: four blocks in flight per step, for models whose targets
  * give fewer chunks (of at least 2W residues) than the chip holds wavefronts */
@@ -966,7 +1091,7 @@ __global__ void __launch_bounds__(64 * WPB_MAX) __attribute__((amdgpu_waves_per_
 /* rows per state visit of Inside with the LDS table (one 16-wavefront
  * workgroup per CU: registers for four rows) */
 #define INFX_NR_INSIDE_LDS 4
-static_assert(INFX_NR_INSIDE_LDS <= INFX_NR_MAX, "BEGL deck rings allow for INFX_NR_MAX rows per visit");
+static_assert(INFX_NR_INSIDE_LDS <= INFX_NR_MAX && INFX_NR_CYK <= INFX_NR_MAX, "BEGL deck rings allow for INFX_NR_MAX rows per visit");
 
 template <int S, bool VSC>
 __global__ void __launch_bounds__(64 * WPB_MAX) std_inside_lds_kernel(const ScanArgs *a)
@@ -1018,15 +1143,17 @@ static const void *std_kernel_ptr(int semi, int want_vsc, int tbl_lds, int gacc,
  * and NR rows of local-begin maxima per wavefront (batched launches: 2) */
 extern "C" int infx_std_rows(int semi, int tbl_lds, int wide)
 {
-  return (semi != SEMI_CYK && tbl_lds && !wide) ? INFX_NR_INSIDE_LDS : 2;
+  if (wide) return 2;
+  if (semi == SEMI_CYK) return INFX_NR_CYK;
+  return tbl_lds ? INFX_NR_INSIDE_LDS : 2;
 }
 
 /* LDS bytes of a workgroup of `wpb` wavefronts (gacc: accumulators in the
  * global slot, LDS holds only the Inside log-sum table) */
-extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc)
+extern "C" size_t infx_std_lds_bytes(int semi, int tbl_lds, int wpb, int acc_n, int gacc, int wide)
 {
   return (semi != SEMI_CYK && tbl_lds ? (size_t) INFX_LOGSUM_PAD * 4 : 0) +
-         (gacc ? 0 : (size_t) wpb * acc_n * 4 * infx_std_rows(semi, tbl_lds, 0));   /* NR rows of 4-byte maxima */
+         (gacc ? 0 : (size_t) wpb * acc_n * 4 * infx_std_rows(semi, tbl_lds, wide));   /* NR rows of 4-byte maxima */
 }
 
 /* resident workgroups per CU for this launch shape (0 if it cannot run) */
@@ -1034,7 +1161,7 @@ extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, 
 {
   const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds, gacc, wide);
   if (!f || wpb < 1 || wpb > WPB_MAX) return 0;
-  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, wpb, acc_n, gacc);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, wpb, acc_n, gacc, wide);
   if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * wpb, lds) != hipSuccess) { (void) hipGetLastError(); return 0; }
@@ -1046,7 +1173,7 @@ extern "C" hipError_t infx_launch_std(int semi, int want_vsc, int tbl_lds, int g
 {
   const void *f = std_kernel_ptr(semi, want_vsc, tbl_lds, gacc, wide);
   if (!f || block.x % 64 || block.x / 64 > WPB_MAX) return hipErrorInvalidValue;
-  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n, gacc);
+  const size_t lds = infx_std_lds_bytes(semi, tbl_lds, block.x / 64, acc_n, gacc, wide);
   hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
   if (e != hipSuccess) return e;
   void *args[] = { (void *) &a };
