@@ -450,6 +450,11 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
     val = [(a - ntrain, b - ntrain) for a, b, _ in train_hits if min(a, b) > ntrain]
     per_hit = np.array([tl[min(a, b) - 1:max(a, b)].max() for a, b in val]) if val else np.zeros(1)
     tau = float(np.quantile(per_hit, args.gate_quantile)) - args.gate_margin
+    # widening of the flagged residues: W - 1, or set on the held-out hits
+    if args.gate_widen == "auto":
+        widen = gt.fit_widen(tl, tau, val, W)
+    else:
+        widen = W - 1 if args.gate_widen == "w" else int(args.gate_widen)
     # the gated pass on the benchmark target
     ia.FastFInsideScan(cm, 2, dsq, 1, min(n, 200000), T, want_hits=True)    # warm profile
     net.score(codes[:100000])
@@ -462,20 +467,20 @@ def gate_leg(args, ia, synth, cm_path, codes, dsq, plain_hits, plain_s):
         tq = float(np.quantile(per_hit, q)) - args.gate_margin
         log("gate: held-out quantile %.3f -> tau %.2f: residues above %.4f, test hits reachable %.4f"
             % (q, tq, float((logits > tq).mean()), float((test_max > tq).mean())))
-    wins = gt.select_windows(logits, tau, W)
+    wins = gt.select_windows(logits, tau, W, widen)
     t1 = time.perf_counter()
     ghits = gt.gated_inside(cm, dsq, wins, T)
     t_inside = time.perf_counter() - t1
     t_gate = t_score + (t1 - t0 - t_score) + t_inside
     rec = gt.recall(keep, ghits)
-    log("gate: train %.1f s (%d steps, loss %.4f -> %.4f), tau %.2f, %d windows (%.2f%% of residues), "
-        "score %.2f s, inside %.2f s, recall %.4f of %d hits" % (t_train, args.gate_steps, losses[0], losses[-1], tau,
+    log("gate: train %.1f s (%d steps, loss %.4f -> %.4f), tau %.2f, widen %d, %d windows (%.2f%% of residues), "
+        "score %.2f s, inside %.2f s, recall %.4f of %d hits" % (t_train, args.gate_steps, losses[0], losses[-1], tau, widen,
                                                                len(wins), 100.0 * sum(j - i + 1 for i, j in wins) / n,
                                                                t_score, t_inside, rec, len(keep)))
     return {"speedup": round(plain_s / t_gate, 2), "recall": round(rec, 5), "plain_hits_e001": len(keep),
             "score_threshold_bits": round(T, 2), "gate_s": round(t_gate, 3), "plain_s": round(plain_s, 2),
             "gate_score_s": round(t_score, 3), "gated_inside_s": round(t_inside, 3), "windows": len(wins),
-            "residue_fraction": round(sum(j - i + 1 for i, j in wins) / n, 5), "tau": round(tau, 3),
+            "residue_fraction": round(sum(j - i + 1 for i, j in wins) / n, 5), "tau": round(tau, 3), "widen": int(widen),
             "train": {"target_mb": ntr / 1e6, "homologs": nh, "steps": args.gate_steps, "seconds": round(t_train, 1),
                       "label_seconds": round(t_data, 1), "labels": label_rule, "positive_fraction": round(pos, 5),
                       "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),
@@ -597,6 +602,8 @@ def main():
     ap.add_argument("--gate-smooth", type=int, default=128, help="residues the gate's logits are averaged over")
     ap.add_argument("--gate-margin", type=float, default=0.5, help="logit margin below the threshold quantile")
     ap.add_argument("--gate-quantile", type=float, default=0.005, help="held-out hit quantile the threshold sits at")
+    ap.add_argument("--gate-widen", default="auto",
+                    help="residues each flagged residue is widened by: 'auto' (set on the held-out hits), 'w' (W-1) or a number")
     ap.add_argument("--gate-labels", choices=("cyk", "inside"), default="cyk",
                     help="training labels: GPU CYK hits at the CYK filter cutoff, or Inside hits")
     ap.add_argument("--cyk1500-mb", type=float, default=0.0,

@@ -202,6 +202,9 @@ template <> struct SRB<SEMI_IINSIDE> {
 #define SEMI_GACC 8
 template <int S> struct SR : SRB<S & 7> {
   static const bool GACC = (S & SEMI_GACC) != 0;
+  /* loads of a visit's rows share their lane offsets (Inside), or each row
+   * computes its own (CYK) */
+  static const bool SHARED_OFF = (S & 7) != SEMI_CYK;
 };
 
 /* Combine child terms c[0..CN-1] and the init term in the reference's
@@ -313,22 +316,44 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
   const T bsc = R::bits(s.beginsc);
   const int bdn = max(rdn, dn);
   float vmax = IMPF;
-  int d[NB], dy[NR][NB], xl[NR][NB];
+  int d[NB], xl[NR][NB];
   bool a[NR][NB];
   T c[NR][NB][CN];
+  int dxm = dx[0];
+#pragma unroll
+  for (int t = 1; t < NR; t++) dxm = max(dxm, dx[t]);
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     d[b] = d0 + 64 * b + lane;
+    /* Inside (SR<S>::SHARED_OFF): offsets shared by the rows, each row
+     * adding its own wave-uniform base.  Every lane reads its own cell
+     * d - sd: past a row's band the cells belong to other states, the next
+     * ring row or the guard after the rows (d <= d0 + 63, and the guard holds
+     * more than 64 cells), so every read lies in the slot; such values are
+     * never stored.  Left residue j+t-d+1: lanes past the widest row's band
+     * read at that band's edge, at most NR-1 residues before the chunk's
+     * first row (inside the 64 zero bytes before the target at its start).
+     * CYK: per-row offsets, idle lanes reading the stand-in cell dn and
+     * residue j+t (fewer wave-uniform values live: CYK's few operations per
+     * load do not hide the scalar spills of per-row bases). */
+    const uint32_t dyb = (uint32_t) (d[b] - sd);
+    const uint32_t xo = (uint32_t) (dsq_w - min(d[b], dxm));
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       a[t][b] = d[b] <= dx[t];
-      dy[t][b] = (a[t][b] ? d[b] : dn) - sd;   /* idle lanes read a valid cell */
-      /* left residue j+t-d+1 (idle lanes: residue j+t, inside the target even
-       * where the row's band is still empty at the start of a chunk) */
-      xl[t][b] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t][b] ? d[b] : 1)));
+      if (R::SHARED_OFF) {
+        xl[t][b] = (int) ldu(dsq_b0 + t, xo);
+        const typename R::T *rb = ibase + ioff[t];
 #pragma unroll
-      for (int k = 0; k < CN; k++)
-        c[t][b][k] = (IR && t > 0 && k == 0) ? R::imp() : ldu(ibase, (uint32_t) (ioff[t] + cb[k] + dy[t][b]));
+        for (int k = 0; k < CN; k++)
+          c[t][b][k] = (IR && t > 0 && k == 0) ? R::imp() : ldu(rb, (uint32_t) cb[k] + dyb);
+      } else {
+        const int dy = (a[t][b] ? d[b] : dn) - sd;
+        xl[t][b] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t][b] ? d[b] : 1)));
+#pragma unroll
+        for (int k = 0; k < CN; k++)
+          c[t][b][k] = (IR && t > 0 && k == 0) ? R::imp() : ldu(ibase, (uint32_t) (ioff[t] + cb[k] + dy));
+      }
     }
   }
 #pragma unroll
@@ -372,7 +397,8 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
 #pragma unroll
     for (int t = 0; t < NR; t++)
       if (a[t][b]) {
-        stu(obase, (uint32_t) (ooff[t] + s.base + d[b]), sc[t][b]);
+        if (R::SHARED_OFF) stu(obase + ooff[t], (uint32_t) (s.base + d[b]), sc[t][b]);
+        else               stu(obase, (uint32_t) (ooff[t] + s.base + d[b]), sc[t][b]);
         begin_update<S>(acc0 + t * acc_n, rdn, hb && d[b] >= bdn && d[b] <= bdx[t], d[b], sc[t][b], bsc);
         if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(sc[t][b]));
       }
@@ -484,16 +510,27 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
   for (int d0 = dn; d0 <= dxm; d0 += 64) {
     const int d = d0 + lane;
     bool a[NR];
-    int dd[NR], xl[NR];
+    int xl[NR];
     T c[NR][CN];
+    /* Inside: shared offsets, reads past a row's band staying in the slot
+     * and residue reads inside the target's pads, as in gen_blocks; CYK:
+     * idle lanes mirror a valid cell and read residue j+t */
+    const uint32_t xo = (uint32_t) (dsq_w - min(d, dxm));
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       a[t] = d <= dx[t];
-      dd[t] = a[t] ? d : max(dn, dx[t]);   /* idle lanes mirror a valid cell */
-      xl[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t] ? d : 1)));   /* idle lanes: residue j+t */
       c[t][0] = R::imp();
+      if (R::SHARED_OFF) {
+        xl[t] = (int) ldu(dsq_b0 + t, xo);
+        const T *rb = rows + roff[t];
 #pragma unroll
-      for (int k = 1; k < CN; k++) c[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + dd[t] - 1));
+        for (int k = 1; k < CN; k++) c[t][k] = ldu(rb, (uint32_t) (cb[k] + d - 1));
+      } else {
+        const int dd = a[t] ? d : max(dn, dx[t]);
+        xl[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t] ? d : 1)));
+#pragma unroll
+        for (int k = 1; k < CN; k++) c[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + dd - 1));
+      }
     }
 #pragma unroll
     for (int t = 0; t < NR; t++) {
@@ -531,7 +568,8 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       if (a[t]) {
-        stu(rows, (uint32_t) (roff[t] + s.base + d), x[t]);
+        if (R::SHARED_OFF) stu(rows + roff[t], (uint32_t) (s.base + d), x[t]);
+        else               stu(rows, (uint32_t) (roff[t] + s.base + d), x[t]);
         begin_update<S>(acc0 + t * acc_n, rdn, hb && d >= bdn && d <= bdx[t], d, x[t], bsc);
         if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(x[t]));
       }
@@ -593,18 +631,20 @@ __device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typenam
     bool aA[NR], aB[NR];
     int xlA[NR], xlB[NR];
     T cA[NR][CN], cB[NR][CN];
+    /* shared offsets, as in il_state (reads up to 127 cells past a band:
+     * the guard after the rows holds more than 128) */
+    const uint32_t xoA = (uint32_t) (dsq_w - min(dA, dxm)), xoB = (uint32_t) (dsq_w - min(dB, dxm));
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       aA[t] = dA <= dx[t]; aB[t] = dB <= dx[t];
-      const int sv = max(dn, dx[t]);                      /* idle lanes mirror a valid cell */
-      const int ddA = aA[t] ? dA : sv, ddB = aB[t] ? dB : sv;
-      xlA[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (aA[t] ? dA : 1)));   /* idle lanes: residue j+t */
-      xlB[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (aB[t] ? dB : 1)));
+      xlA[t] = (int) ldu(dsq_b0 + t, xoA);
+      xlB[t] = (int) ldu(dsq_b0 + t, xoB);
       cA[t][0] = R::imp(); cB[t][0] = R::imp();
+      const T *rb = rows + roff[t];
 #pragma unroll
       for (int k = 1; k < CN; k++) {
-        cA[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + ddA - 1));
-        cB[t][k] = ldu(rows, (uint32_t) (roff[t] + cb[k] + ddB - 1));
+        cA[t][k] = ldu(rb, (uint32_t) (cb[k] + dA - 1));
+        cB[t][k] = ldu(rb, (uint32_t) (cb[k] + dB - 1));
       }
     }
 #pragma unroll
@@ -640,12 +680,12 @@ __device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typenam
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       if (aA[t]) {
-        stu(rows, (uint32_t) (roff[t] + s.base + dA), xA[t]);
+        stu(rows + roff[t], (uint32_t) (s.base + dA), xA[t]);
         begin_update<S>(acc0 + t * acc_n, rdn, hb && dA >= bdn && dA <= bdx[t], dA, xA[t], bsc);
         if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(xA[t]));
       }
       if (aB[t]) {
-        stu(rows, (uint32_t) (roff[t] + s.base + dB), xB[t]);
+        stu(rows + roff[t], (uint32_t) (s.base + dB), xB[t]);
         begin_update<S>(acc0 + t * acc_n, rdn, hb && dB >= bdn && dB <= bdx[t], dB, xB[t], bsc);
         if (VSC && vs[t]) vmax = fmaxf(vmax, R::score(xB[t]));
       }

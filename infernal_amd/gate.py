@@ -13,7 +13,8 @@ stretches of a target go to the exact Inside scan:
   smooth             the logits averaged over 64 residues
   select_windows     residues whose smoothed logit passes the threshold, widened by
                      W-1 on each side (a hit of length <= W around them fits)
-                     and merged, as the pipeline widens CYK survivors
+                     and merged, as the pipeline widens CYK survivors, or by
+                     the widening fit_widen sets on held-out hits
   gated_inside       exact FastFInsideScan on those windows only
 
 The plain scan (CYK and Inside over everything) stays the fallback and the
@@ -257,20 +258,56 @@ def labels_from_hits(n, hits):
     return lab
 
 
-def select_windows(logits, tau, W):
-    """Windows (1-based, inclusive) around residues with logit > tau."""
+def select_windows(logits, tau, W, widen=None):
+    """Windows (1-based, inclusive) around residues with logit > tau, each
+    widened by `widen` residues on each side (default W - 1: any hit of
+    length <= W that touches a flagged residue fits) and merged."""
     n = len(logits)
+    m = W - 1 if widen is None else int(min(max(widen, 0), W - 1))
     idx = np.nonzero(logits > tau)[0] + 1
     if len(idx) == 0:
         return []
-    lo = np.maximum(1, idx - (W - 1))
-    hi = np.minimum(n, idx + (W - 1))
+    lo = np.maximum(1, idx - m)
+    hi = np.minimum(n, idx + m)
     # a new window starts where a residue's window does not touch the previous one
     brk = np.nonzero(lo[1:] > np.maximum.accumulate(hi)[:-1] + 1)[0] + 1
     starts = np.concatenate([[0], brk])
     ends = np.concatenate([brk, [len(idx)]])
     hmax = np.maximum.accumulate(hi)
     return [(int(lo[a]), int(hmax[b - 1])) for a, b in zip(starts, ends)]
+
+
+def windows_cover(windows, hits):
+    """For each hit (start, stop, ...), whether one window holds all of it."""
+    if not windows:
+        return np.zeros(len(hits), dtype=bool)
+    ws = np.array([a for a, _ in windows], dtype=np.int64)
+    we = np.array([b for _, b in windows], dtype=np.int64)
+    out = np.zeros(len(hits), dtype=bool)
+    for i, h in enumerate(hits):
+        lo, hi = min(h[0], h[1]), max(h[0], h[1])
+        k = int(np.searchsorted(ws, lo, side="right")) - 1
+        out[i] = k >= 0 and we[k] >= hi
+    return out
+
+
+def fit_widen(logits, tau, hits, W, slack=0.002):
+    """The widening the gate's windows need, set on held-out hits as the
+    threshold is: the smallest of 32, 40, 50, ... (each step 1.25 times the
+    last, rounded up; or W - 1) whose windows hold whole as many held-out
+    hits as W - 1 does, less `slack` of them.
+    The gate's labels are the residues inside hits, so the flagged residues
+    already span most of a hit; the widening covers the ends the smoothed
+    logits miss."""
+    if not hits:
+        return W - 1
+    full = windows_cover(select_windows(logits, tau, W), hits).mean()
+    m = 32
+    while m < W - 1:
+        if windows_cover(select_windows(logits, tau, W, m), hits).mean() >= full - slack:
+            return m
+        m = -(-m * 5 // 4)
+    return W - 1
 
 
 def gated_inside(cm, dsq, windows, cutoff, qdbidx=2):

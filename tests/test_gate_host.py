@@ -57,3 +57,53 @@ def test_smooth_is_a_centred_mean():
     for t in (0, 5, 100, 299):
         lo, hi = max(0, t - 8), min(300, t + 8)
         assert abs(s[t] - x[lo:hi].mean()) < 1e-5
+
+
+def _windows_brute_m(logits, tau, m):
+    """_windows_brute with an explicit widening m (W - 1 = m)."""
+    return _windows_brute(logits, tau, m + 1)
+
+
+def test_select_windows_widening_and_cover():
+    rng = np.random.default_rng(3)
+    for trial in range(40):
+        n = int(rng.integers(1, 400))
+        logits = rng.standard_normal(n).astype(np.float32)
+        tau = float(rng.uniform(0.5, 3.0))
+        W = int(rng.integers(2, 60))
+        m = int(rng.integers(0, W))
+        wins = g.select_windows(logits, tau, W, m)
+        assert wins == _windows_brute_m(logits, tau, m), (n, tau, W, m)
+        assert g.select_windows(logits, tau, W, 10 * W) == g.select_windows(logits, tau, W)   # capped at W - 1
+        hits = []
+        for _ in range(20):
+            a = int(rng.integers(1, n + 1)); b = int(rng.integers(a, min(n, a + 30) + 1))
+            hits.append((a, b) if rng.random() < 0.5 else (b, a))
+        got = g.windows_cover(wins, hits)
+        for h, ok in zip(hits, got):
+            lo, hi = min(h), max(h)
+            assert ok == any(s <= lo and hi <= e for s, e in wins)
+
+
+def test_fit_widen_takes_the_smallest_grid_widening_that_keeps_the_hits():
+    n, W = 8000, 1500
+    logits = np.zeros(n, dtype=np.float32)
+    hits = []
+    # hits whose flagged core misses up to 40 residues at either end
+    for k, start in enumerate(range(100, 4800, 400)):
+        a, b = start, start + 200
+        logits[a - 1 + 40 - (k % 3) * 10:b - 40 + (k % 2) * 10] = 5.0
+        hits.append((a, b))
+    grid = [32]
+    while grid[-1] < W - 1:
+        grid.append(-(-grid[-1] * 5 // 4))
+    assert grid[:4] == [32, 40, 50, 63]
+    assert g.fit_widen(logits, 1.0, hits, W) == 40           # 32 misses ends 40 away, 40 holds every hit
+    assert g.windows_cover(g.select_windows(logits, 1.0, W, 40), hits).all()
+    assert not g.windows_cover(g.select_windows(logits, 1.0, W, 39), hits).all()
+    assert g.fit_widen(logits, 1.0, [], W) == W - 1
+    # a hit more than W - 1 from every flagged residue (no window holds it at
+    # any widening) does not force W - 1; one that only a wide window
+    # reaches (the last core ends at 4670) takes the first grid step >= 920
+    assert g.fit_widen(logits, 1.0, hits + [(7800, 7890)], W) == 40
+    assert g.fit_widen(logits, 1.0, hits + [(5500, 5590)], W) == min(x for x in grid if x >= 920)

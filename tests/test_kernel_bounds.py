@@ -8,10 +8,13 @@ overshoot, local-begin accumulators, root, emission rows, residue reads) are
 evaluated for every state over the chunk's warm-up and steady rows and at the
 target's end.  Every access must fall inside its buffer:
 
-  residues  device bytes [64, L + 128): the target itself and the 64 zero
-            bytes after it (the 64 bytes before it are a guard the kernel
-            must not rely on)
-  rows      [0, R) of each ring row (the kernel adds the ring row's offset)
+  residues  device bytes [61, L + 128): the target itself, the 64 zero
+            bytes after it, and up to NR-1 = 3 of the 64 zero bytes before
+            it (lanes past a visit's widest band read at its edge, at most
+            NR-1 residues before a chunk's first row)
+  rows      stores [0, R) of each ring row (the kernel adds the ring row's
+            offset); reads [0, R + G): every lane of a block reads its own
+            cell, which past the last ring row lies in the guard after it
   decks     [-G, RB + G) with G = W + 128 rounded up to 64 (guard cells)
   LDS       accumulator index [0, acc_n)
   emission  [0, nesc)
@@ -82,11 +85,16 @@ class Checker:
     def row(self, idx, ctx):
         self.rng("row", idx, 0, self.R, ctx)
 
+    def rowread(self, idx, ctx):
+        """A read by every lane of a block: past a row's end it lands in the
+        next ring row or, after the last, in the guard of G cells."""
+        self.rng("row read", idx, 0, self.R + self.G, ctx)
+
     def deck(self, idx, ctx):
         self.rng("deck", idx, -self.G, self.RB + self.G, ctx)
 
     def dsq(self, idx, ctx):
-        self.rng("dsq", idx, 64, self.L + 128, ctx)
+        self.rng("dsq", idx, 64 - (NR_MAX - 1), self.L + 128, ctx)
 
     def acc(self, idx, ctx):
         self.rng("acc", idx, 0, self.acc_n, ctx)
@@ -127,6 +135,9 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
             chk.esc(aux + (yy * KP + xj if em == EMITPAIR else yy), ctx + " ev")
     cbase = int(np.int32(w[12]))
     cb = [cbase + s16(int(w[13 + (k >> 1)]) >> (16 * (k & 1))) for k in range(6)]
+    # the widest row of a visit of NR_MAX rows at this position (residue reads
+    # of lanes past it are clamped to it)
+    dxv = min(jp0 + NR_MAX - 1, sdx)
     if kind in (KIND_GENERIC, KIND_BEGL):
         rmask, rstride = aux & 0xffff, aux >> 16
         ir = kind == KIND_GENERIC and seq
@@ -138,14 +149,11 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
             for d0 in range(dn, max(x0, x1) + 1, 64):
                 d = d0 + LANES
                 a0, a1 = d <= x0, d <= x1
-                dy0 = np.where(a0, d, dn) - sd
-                dy1 = np.where(a1, d, dn) - sd
-                chk.dsq(dsq0(np.where(a0, d, 1)), ctx + " xl0")
-                chk.dsq(dsq1(np.where(a1, d, 1)), ctx + " xl1")
+                # every lane reads its own d (offsets shared by the rows)
+                chk.dsq(dsq0(np.minimum(d, dxv)), ctx + " xl0")
+                chk.dsq(dsq1(np.minimum(d, dxv)), ctx + " xl1")
                 for k in range(cnum):
-                    chk.row(cb[k] + dy0, ctx + " child0")
-                    if not (ir and k == 0):
-                        chk.row(cb[k] + dy1, ctx + " child1")
+                    chk.rowread(cb[k] + d - sd, ctx + " child")
                 if kind == KIND_BEGL:
                     chk.deck(((jp0 & rmask) * rstride + base + d)[a0], ctx + " begl out0")
                     chk.deck(((jp1 & rmask) * rstride + base + d)[a1], ctx + " begl out1")
@@ -157,16 +165,16 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
                         chk.acc((d - rdn)[a1 & (d >= bdn) & (d <= bdx1)], ctx + " acc1")
     elif kind == KIND_IL:
         chk.row(np.array([base + dn - 1]), ctx + " carry")
+        # FInside bands wider than one block take 128-cell blocks, two cells
+        # per lane: every lane reads up to d0 + 127
         for d0 in range(dn, max(dx0, dx1) + 1, 64):
+            for dw in ((d0 + LANES, d0 + 64 + LANES) if max(dx0, dx1) - dn >= 64 else (d0 + LANES,)):
+                chk.dsq(dsq0(np.minimum(dw, dxv)), ctx + " xl0")
+                chk.dsq(dsq1(np.minimum(dw, dxv)), ctx + " xl1")
+                for k in range(1, cnum):
+                    chk.rowread(cb[k] + dw - 1, ctx + " child")
             d = d0 + LANES
             a0, a1 = d <= dx0, d <= dx1
-            dd0 = np.where(a0, d, max(dn, dx0))
-            dd1 = np.where(a1, d, max(dn, dx1))
-            chk.dsq(dsq0(np.where(a0, d, 1)), ctx + " xl0")
-            chk.dsq(dsq1(np.where(a1, d, 1)), ctx + " xl1")
-            for k in range(1, cnum):
-                chk.row(cb[k] + dd0 - 1, ctx + " child0")
-                chk.row(cb[k] + dd1 - 1, ctx + " child1")
             chk.row((base + d)[a0], ctx + " out0")
             chk.row((base + d)[a1], ctx + " out1")
             if hb:
@@ -311,15 +319,20 @@ def test_begl_rings_hold_one_visit(name):
 
 
 def test_checker_catches_planted_overreach():
-    """The checker itself: residue j0 is fine, but the left residue of the
-    idle-lane stand-in d = dn on the first row of a chunk (target position
-    j0 - dn + 1 < 1) is reported."""
+    """The checker itself: on the first row of a chunk at the target's start
+    (j0 = 1), a lane reading the left residue of d = NR_MAX + 1 (target
+    position 1 - NR_MAX, NR_MAX bytes before the target) is reported, while
+    the NR_MAX - 1 bytes the clamped reads may touch are not; a row read one
+    cell past the guard is reported."""
     chk = Checker(R=10, RB=10, nesc=10, W=5, L=5, acc_n=3)
-    j0, dn = 1, 3
-    chk.dsq(np.array([64 + j0 - 1]), "residue j0")
+    j0 = 1
+    chk.dsq(np.array([64 + j0 - 1, 64 + j0 - NR_MAX]), "residue j0, and NR_MAX - 1 before it")
+    chk.rowread(np.array([0, 10 + chk.G - 1]), "row reads up to the guard's end")
     assert not chk.errors
-    chk.dsq(np.array([64 + j0 - dn]), "planted")
-    assert chk.errors
+    chk.dsq(np.array([64 + j0 - 1 - NR_MAX]), "planted")
+    assert len(chk.errors) == 1
+    chk.rowread(np.array([10 + chk.G]), "planted")
+    assert len(chk.errors) == 2
 
 
 def _blocks(dn, dxm, nbw):
