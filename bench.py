@@ -592,7 +592,7 @@ def main():
     ap.add_argument("--cm", default=os.path.join(ROOT, "tests", "golden", "cm", "synth_multiloop350.cm"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=8000, help="residues per CPU-baseline window")
+    ap.add_argument("--cpu-sample", type=int, default=12000, help="residues per CPU-baseline window (16 windows: ~12 s of CPU work)")
     ap.add_argument("--only", choices=["both", "cyk", "finside"], default="both")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--gate-steps", type=int, default=6000, help="learned-gate training steps (0 = skip the gate leg)")

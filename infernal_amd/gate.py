@@ -293,21 +293,23 @@ def windows_cover(windows, hits):
 
 def fit_widen(logits, tau, hits, W, slack=0.002):
     """The widening the gate's windows need, set on held-out hits as the
-    threshold is: the smallest of 32, 40, 50, ... (each step 1.25 times the
-    last, rounded up; or W - 1) whose windows hold whole as many held-out
-    hits as W - 1 does, less `slack` of them.
+    threshold is: the smallest widening in 0 .. W - 1 whose windows hold
+    whole as many held-out hits as W - 1 does, less `slack` of them (found
+    by bisection: a wider window holds every hit a narrower one holds).
     The gate's labels are the residues inside hits, so the flagged residues
     already span most of a hit; the widening covers the ends the smoothed
     logits miss."""
     if not hits:
         return W - 1
     full = windows_cover(select_windows(logits, tau, W), hits).mean()
-    m = 32
-    while m < W - 1:
+    lo, hi = 0, W - 1          # the answer lies in [lo, hi]; hi always qualifies
+    while lo < hi:
+        m = (lo + hi) // 2
         if windows_cover(select_windows(logits, tau, W, m), hits).mean() >= full - slack:
-            return m
-        m = -(-m * 5 // 4)
-    return W - 1
+            hi = m
+        else:
+            lo = m + 1
+    return hi
 
 
 def gated_inside(cm, dsq, windows, cutoff, qdbidx=2):
