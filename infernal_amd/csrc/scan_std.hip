@@ -326,18 +326,18 @@ __device__ __forceinline__ float gen_blocks(SW_PARAMS, const DState &s,
   for (int b = 0; b < NB; b++) {
     d[b] = d0 + 64 * b + lane;
     /* Inside (SR<S>::SHARED_OFF): offsets shared by the rows, each row
-     * adding its own wave-uniform base.  Every lane reads its own cell
-     * d - sd: past a row's band the cells belong to other states, the next
-     * ring row or the guard after the rows (d <= d0 + 63, and the guard holds
-     * more than 64 cells), so every read lies in the slot; such values are
-     * never stored.  Left residue j+t-d+1: lanes past the widest row's band
-     * read at that band's edge, at most NR-1 residues before the chunk's
-     * first row (inside the 64 zero bytes before the target at its start).
-     * CYK: per-row offsets, idle lanes reading the stand-in cell dn and
-     * residue j+t (fewer wave-uniform values live: CYK's few operations per
-     * load do not hide the scalar spills of per-row bases). */
-    const uint32_t dyb = (uint32_t) (d[b] - sd);
-    const uint32_t xo = (uint32_t) (dsq_w - min(d[b], dxm));
+     * adding its own wave-uniform base.  Lanes past the widest row's band
+     * read at that band's edge (cell dxm - sd of each child, inside its
+     * storage; residue j+t-dxm+1, at most NR-1 residues before the chunk's
+     * first row: inside the 64 zero bytes before the target at its start),
+     * so they repeat an active lane's addresses; lanes of a narrower row
+     * read that row's stored out-of-band cells.  Such values are never
+     * stored.  CYK: per-row offsets, idle lanes reading the stand-in cell dn
+     * and residue j+t (fewer wave-uniform values live: CYK's few operations
+     * per load do not hide the scalar spills of per-row bases). */
+    const int dcl = min(d[b], dxm);
+    const uint32_t dyb = (uint32_t) (dcl - sd);
+    const uint32_t xo = (uint32_t) (dsq_w - dcl);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       a[t][b] = d[b] <= dx[t];
@@ -512,10 +512,11 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
     bool a[NR];
     int xl[NR];
     T c[NR][CN];
-    /* Inside: shared offsets, reads past a row's band staying in the slot
-     * and residue reads inside the target's pads, as in gen_blocks; CYK:
-     * idle lanes mirror a valid cell and read residue j+t */
-    const uint32_t xo = (uint32_t) (dsq_w - min(d, dxm));
+    /* Inside: shared offsets, lanes past the widest band reading at its
+     * edge, as in gen_blocks; CYK: idle lanes mirror a valid cell and read
+     * residue j+t */
+    const int dcl = min(d, dxm);
+    const uint32_t xo = (uint32_t) (dsq_w - dcl);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       a[t] = d <= dx[t];
@@ -524,7 +525,7 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
         xl[t] = (int) ldu(dsq_b0 + t, xo);
         const T *rb = rows + roff[t];
 #pragma unroll
-        for (int k = 1; k < CN; k++) c[t][k] = ldu(rb, (uint32_t) (cb[k] + d - 1));
+        for (int k = 1; k < CN; k++) c[t][k] = ldu(rb, (uint32_t) (cb[k] + dcl - 1));
       } else {
         const int dd = a[t] ? d : max(dn, dx[t]);
         xl[t] = (int) ldu(dsq_b0 + t, (uint32_t) (dsq_w - (a[t] ? d : 1)));
@@ -631,9 +632,10 @@ __device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typenam
     bool aA[NR], aB[NR];
     int xlA[NR], xlB[NR];
     T cA[NR][CN], cB[NR][CN];
-    /* shared offsets, as in il_state (reads up to 127 cells past a band:
-     * the guard after the rows holds more than 128) */
-    const uint32_t xoA = (uint32_t) (dsq_w - min(dA, dxm)), xoB = (uint32_t) (dsq_w - min(dB, dxm));
+    /* shared offsets, lanes past the widest band reading at its edge, as in
+     * il_state */
+    const int dcA = min(dA, dxm), dcB = min(dB, dxm);
+    const uint32_t xoA = (uint32_t) (dsq_w - dcA), xoB = (uint32_t) (dsq_w - dcB);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       aA[t] = dA <= dx[t]; aB[t] = dB <= dx[t];
@@ -643,8 +645,8 @@ __device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typenam
       const T *rb = rows + roff[t];
 #pragma unroll
       for (int k = 1; k < CN; k++) {
-        cA[t][k] = ldu(rb, (uint32_t) (cb[k] + dA - 1));
-        cB[t][k] = ldu(rb, (uint32_t) (cb[k] + dB - 1));
+        cA[t][k] = ldu(rb, (uint32_t) (cb[k] + dcA - 1));
+        cB[t][k] = ldu(rb, (uint32_t) (cb[k] + dcB - 1));
       }
     }
 #pragma unroll

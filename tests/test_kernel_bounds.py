@@ -12,9 +12,9 @@ target's end.  Every access must fall inside its buffer:
             bytes after it, and up to NR-1 = 3 of the 64 zero bytes before
             it (lanes past a visit's widest band read at its edge, at most
             NR-1 residues before a chunk's first row)
-  rows      stores [0, R) of each ring row (the kernel adds the ring row's
-            offset); reads [0, R + G): every lane of a block reads its own
-            cell, which past the last ring row lies in the guard after it
+  rows      [0, R) of each ring row (the kernel adds the ring row's offset):
+            lanes past the band read the band's edge cell (Inside) or the
+            stand-in cell dn (CYK)
   decks     [-G, RB + G) with G = W + 128 rounded up to 64 (guard cells)
   LDS       accumulator index [0, acc_n)
   emission  [0, nesc)
@@ -149,11 +149,17 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
             for d0 in range(dn, max(x0, x1) + 1, 64):
                 d = d0 + LANES
                 a0, a1 = d <= x0, d <= x1
-                # every lane reads its own d (offsets shared by the rows)
-                chk.dsq(dsq0(np.minimum(d, dxv)), ctx + " xl0")
-                chk.dsq(dsq1(np.minimum(d, dxv)), ctx + " xl1")
+                # Inside: lanes past the visit's widest band read at its edge
+                # (offsets shared by the rows); CYK: idle lanes read the
+                # stand-in cell dn and residue j+t
+                dcl = np.minimum(d, dxv)
+                chk.dsq(dsq0(dcl), ctx + " xl0")
+                chk.dsq(dsq1(dcl), ctx + " xl1")
+                chk.dsq(dsq1(np.where(a1, d, 1)), ctx + " xl1 cyk")
                 for k in range(cnum):
-                    chk.rowread(cb[k] + d - sd, ctx + " child")
+                    chk.row(cb[k] + dcl - sd, ctx + " child")
+                    chk.row(cb[k] + np.where(a0, d, dn) - sd, ctx + " child0 cyk")
+                    chk.row(cb[k] + np.where(a1, d, dn) - sd, ctx + " child1 cyk")
                 if kind == KIND_BEGL:
                     chk.deck(((jp0 & rmask) * rstride + base + d)[a0], ctx + " begl out0")
                     chk.deck(((jp1 & rmask) * rstride + base + d)[a1], ctx + " begl out1")
@@ -169,10 +175,14 @@ def check_state(chk, w, rdn, rdx0, rdx1, W, jp0, two, j0, local):
         # per lane: every lane reads up to d0 + 127
         for d0 in range(dn, max(dx0, dx1) + 1, 64):
             for dw in ((d0 + LANES, d0 + 64 + LANES) if max(dx0, dx1) - dn >= 64 else (d0 + LANES,)):
-                chk.dsq(dsq0(np.minimum(dw, dxv)), ctx + " xl0")
-                chk.dsq(dsq1(np.minimum(dw, dxv)), ctx + " xl1")
+                dcl = np.minimum(dw, dxv)
+                chk.dsq(dsq0(dcl), ctx + " xl0")
+                chk.dsq(dsq1(dcl), ctx + " xl1")
                 for k in range(1, cnum):
-                    chk.rowread(cb[k] + dw - 1, ctx + " child")
+                    chk.row(cb[k] + dcl - 1, ctx + " child")
+            for dd in (np.where(d0 + LANES <= dx0, d0 + LANES, max(dn, dx0)), np.where(d0 + LANES <= dx1, d0 + LANES, max(dn, dx1))):
+                for k in range(1, cnum):
+                    chk.row(cb[k] + dd - 1, ctx + " child cyk")
             d = d0 + LANES
             a0, a1 = d <= dx0, d <= dx1
             chk.row((base + d)[a0], ctx + " out0")
