@@ -24,6 +24,7 @@ extern "C" int infx_std_rows(int semi, int tbl_lds, int wide);
 extern "C" int infx_std_occupancy(int semi, int want_vsc, int tbl_lds, int wpb, int acc_n, int gacc, int wide);
 extern "C" int infx_std_batch_occupancy(int semi, int wpb, int acc_n);
 extern "C" hipError_t infx_launch_zero_u64(unsigned long long *p, hipStream_t stream);
+extern "C" hipError_t infx_launch_zero_bytes(void *p, size_t n, hipStream_t stream);
 extern "C" hipError_t infx_launch_std_batch(int semi, dim3 grid, dim3 block, hipStream_t stream, const ScanArgs *a, int acc_n);
 
 /* ------------------------------------------------------------------------ */
@@ -793,9 +794,11 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   if ((status = D->d_cand.reserve((size_t) cand_cap * sizeof(Cand))) != eslOK) return infx_fail(eslEMEM, errbuf, "%s: device allocation failed", fname);
   cand_cap = (int64_t) (D->d_cand.cap / sizeof(Cand));
   /* target residues i0..j0 behind and before 64 zero bytes (residue codes
-   * past j0 are read, and ignored, by idle lanes of the last row pair) */
+   * past j0 are read, and ignored, by idle lanes of the last row pair); the
+   * pads are cleared by a kernel, so a call leaves no memset in the stream */
   if (!pinned &&
-      (hipMemsetAsync(D->d_dsq.ptr, 0, (size_t) Lspan + 128, g_stream) != hipSuccess ||
+      (infx_launch_zero_bytes(D->d_dsq.ptr, 64, g_stream) != hipSuccess ||
+       infx_launch_zero_bytes((char *) D->d_dsq.ptr + 64 + Lspan, 64, g_stream) != hipSuccess ||
        hipMemcpyAsync((char *) D->d_dsq.ptr + 64, dsq + i0, (size_t) Lspan, hipMemcpyHostToDevice, g_stream) != hipSuccess))
     return infx_fail(eslESYS, errbuf, "%s: dsq upload failed", fname);
 
@@ -854,7 +857,7 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   float kernel_ms_total = 0;
   int launches = 0;
   for (int attempt = 0; attempt < 3; attempt++) {
-    if (hipMemsetAsync(A.ncand, 0, sizeof(unsigned long long), g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: memset failed", fname);
+    if (infx_launch_zero_u64(A.ncand, g_stream) != hipSuccess) return infx_fail(eslESYS, errbuf, "%s: counter reset failed", fname);
     (void) hipEventRecord(g_ev0, g_stream);   /* timing only */
     if (upload(D->d_args, &A, sizeof(A)) != eslOK) return infx_fail(eslESYS, errbuf, "%s: parameter upload failed", fname);
     hipError_t e = infx_launch_std(semi, A.want_vsc, tbl_lds, gacc, wide, grid, block, g_stream, (const ScanArgs *) D->d_args.ptr, acc_n);

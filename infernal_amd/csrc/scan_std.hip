@@ -449,37 +449,38 @@ __device__ __forceinline__ float gen_state(SW_PARAMS, const DState &s,
  * associatively, (A2, B2) o (A1, B1) = (A2 + A1, lse(B2, A2 + B1)) in log2
  * terms, so a Kogge-Stone prefix over the 64 lanes (DPP row_shr 1, 2, 4, 8
  * within each 16-lane row, then row_bcast 15 and 31 across rows) gives every
- * lane its map from the carry x_{d0-1}.  lse uses the hardware's exp2/log2,
- * not the reference's table, so the result is only close to the reference's
- * values (within a few table steps); the sweeps that follow make every lane
- * exact.  Lanes without a source take the identity (A 0, B far below any DP
- * value), which leaves their map unchanged bit for bit. */
-__device__ __forceinline__ float lse2_hw(float x, float y)
+ * lane its map from the carry x_{d0-1}.  lse is the FLogsum of the LDS table
+ * (seven VALU operations and one LDS read; the hardware's exp2/log2 pair cost
+ * more issue cycles and measured 3% slower over the whole kernel), but the
+ * composition sums in a tree order, so the result is only close to the
+ * reference's values (within a few table steps); the sweeps that follow make
+ * every lane exact.  Lanes without a source take the identity (A 0, B far
+ * below any DP value: the log-sum returns its max unchanged), which leaves
+ * their map unchanged bit for bit. */
+__device__ __forceinline__ float lse_tbl(float x, float y, const float *tbl)
 {
-  const float m = fmaxf(x, y);
-  const float t = __builtin_amdgcn_exp2f(-fabsf(__fsub_rn(x, y)));
-  return __fadd_rn(m, __builtin_amdgcn_logf(__fadd_rn(1.0f, t)));
+  return SRB<SEMI_FINSIDE>::op(x, y, tbl);
 }
 
 template <int CTRL, int ROWS>
-__device__ __forceinline__ void il_scan_step(float &A, float &B)
+__device__ __forceinline__ void il_scan_step(float &A, float &B, const float *tbl)
 {
   const float oA = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(0.0f), __float_as_int(A), CTRL, ROWS, 0xf, false));
   const float oB = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(-1e37f), __float_as_int(B), CTRL, ROWS, 0xf, false));
-  B = lse2_hw(B, __fadd_rn(A, oB));
+  B = lse_tbl(B, __fadd_rn(A, oB), tbl);
   A = __fadd_rn(A, oA);
 }
 
-__device__ __forceinline__ float il_guess_finside(float pre, float e, float t0, float carry)
+__device__ __forceinline__ float il_guess_finside(float pre, float e, float t0, float carry, const float *tbl)
 {
   float A = __fadd_rn(t0, e), B = __fadd_rn(pre, e);
-  il_scan_step<0x111, 0xf>(A, B);   /* row_shr:1 */
-  il_scan_step<0x112, 0xf>(A, B);   /* row_shr:2 */
-  il_scan_step<0x114, 0xf>(A, B);   /* row_shr:4 */
-  il_scan_step<0x118, 0xf>(A, B);   /* row_shr:8 */
-  il_scan_step<0x142, 0xa>(A, B);   /* row_bcast:15 into rows 1 and 3 */
-  il_scan_step<0x143, 0xc>(A, B);   /* row_bcast:31 into rows 2 and 3 */
-  return lse2_hw(B, __fadd_rn(A, carry));
+  il_scan_step<0x111, 0xf>(A, B, tbl);   /* row_shr:1 */
+  il_scan_step<0x112, 0xf>(A, B, tbl);   /* row_shr:2 */
+  il_scan_step<0x114, 0xf>(A, B, tbl);   /* row_shr:4 */
+  il_scan_step<0x118, 0xf>(A, B, tbl);   /* row_shr:8 */
+  il_scan_step<0x142, 0xa>(A, B, tbl);   /* row_bcast:15 into rows 1 and 3 */
+  il_scan_step<0x143, 0xc>(A, B, tbl);   /* row_bcast:31 into rows 2 and 3 */
+  return lse_tbl(B, __fadd_rn(A, carry), tbl);
 }
 
 /* IL state: x_d = op(pre_d, x_{d-1} + t0) + e_d with pre_d the other CN-1
@@ -552,7 +553,7 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
        * defines uniquely (lane k is exact after k + 1 sweeps from any
        * start).  CYK and IInside start from pre + e (the first sweep brings
        * in the carry); FInside from the prefix-scan estimate of the chain. */
-      if ((S & 7) == SEMI_FINSIDE) x[t] = il_guess_finside(pre[t], e[t], t0, carry[t]);
+      if ((S & 7) == SEMI_FINSIDE) x[t] = il_guess_finside(pre[t], e[t], t0, carry[t], (const float *) tbl);
       else                         x[t] = R::add(pre[t], e[t]);
     }
     for (int it = 0; it < 66; it++) {
@@ -589,21 +590,21 @@ __device__ __forceinline__ float il_state(SW_PARAMS, const DState &s, typename S
  * The starting point is il_guess_finside's prefix over the lanes, applied to
  * the composition of each lane's two maps. */
 __device__ __forceinline__ void il_guess_finside2(float preA, float eA, float preB, float eB, float t0, float carry,
-                                                  float &xA, float &xB)
+                                                  float &xA, float &xB, const float *tbl)
 {
   const float a0 = __fadd_rn(t0, eA), b0 = __fadd_rn(preA, eA);
   const float a1 = __fadd_rn(t0, eB), b1 = __fadd_rn(preB, eB);
-  float A = __fadd_rn(a1, a0), B = lse2_hw(b1, __fadd_rn(a1, b0));   /* second cell's map from the lane's input */
-  il_scan_step<0x111, 0xf>(A, B);   /* row_shr:1 */
-  il_scan_step<0x112, 0xf>(A, B);   /* row_shr:2 */
-  il_scan_step<0x114, 0xf>(A, B);   /* row_shr:4 */
-  il_scan_step<0x118, 0xf>(A, B);   /* row_shr:8 */
-  il_scan_step<0x142, 0xa>(A, B);   /* row_bcast:15 into rows 1 and 3 */
-  il_scan_step<0x143, 0xc>(A, B);   /* row_bcast:31 into rows 2 and 3 */
-  const float xlast = lse2_hw(B, __fadd_rn(A, carry));                /* the lane's second cell */
+  float A = __fadd_rn(a1, a0), B = lse_tbl(b1, __fadd_rn(a1, b0), tbl);   /* second cell's map from the lane's input */
+  il_scan_step<0x111, 0xf>(A, B, tbl);   /* row_shr:1 */
+  il_scan_step<0x112, 0xf>(A, B, tbl);   /* row_shr:2 */
+  il_scan_step<0x114, 0xf>(A, B, tbl);   /* row_shr:4 */
+  il_scan_step<0x118, 0xf>(A, B, tbl);   /* row_shr:8 */
+  il_scan_step<0x142, 0xa>(A, B, tbl);   /* row_bcast:15 into rows 1 and 3 */
+  il_scan_step<0x143, 0xc>(A, B, tbl);   /* row_bcast:31 into rows 2 and 3 */
+  const float xlast = lse_tbl(B, __fadd_rn(A, carry), tbl);                /* the lane's second cell */
   const float xin = __int_as_float(shr1(__float_as_int(carry), __float_as_int(xlast)));   /* its input */
-  xA = lse2_hw(b0, __fadd_rn(a0, xin));
-  xB = lse2_hw(b1, __fadd_rn(a1, xA));
+  xA = lse_tbl(b0, __fadd_rn(a0, xin), tbl);
+  xB = lse_tbl(b1, __fadd_rn(a1, xA), tbl);
 }
 
 template <int S, bool VSC, int CN, int NR>
@@ -665,7 +666,7 @@ __device__ __forceinline__ float il_state_w2(SW_PARAMS, const DState &s, typenam
       preB[t] = combine_children<S, CN>(cB[t], iniB, false, tbl);
       eA[t] = pick_esc<S>(ev, xlA[t]);
       eB[t] = pick_esc<S>(ev, xlB[t]);
-      il_guess_finside2(preA[t], eA[t], preB[t], eB[t], t0, carry[t], xA[t], xB[t]);
+      il_guess_finside2(preA[t], eA[t], preB[t], eB[t], t0, carry[t], xA[t], xB[t], (const float *) tbl);
     }
     for (int it = 0; it < 66; it++) {
       bool changed = false;
@@ -1272,6 +1273,20 @@ __global__ void zero_u64_kernel(unsigned long long *p)
 extern "C" hipError_t infx_launch_zero_u64(unsigned long long *p, hipStream_t stream)
 {
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, stream, p);
+  return hipGetLastError();
+}
+
+/* n <= 4096 bytes set to zero by a kernel: the zero pads around an uploaded
+ * target (the scanners issue no memset nodes into their stream) */
+__global__ void zero_bytes_kernel(unsigned char *p, unsigned n)
+{
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+extern "C" hipError_t infx_launch_zero_bytes(void *p, size_t n, hipStream_t stream)
+{
+  if (n == 0) return hipSuccess;
+  if (n > 4096) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(64), 0, stream, (unsigned char *) p, (unsigned) n);
   return hipGetLastError();
 }
 

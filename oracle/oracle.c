@@ -228,7 +228,8 @@ static int hitacc_finish(HITACC *a, ORACLE_RESULT *res, char *errbuf)
   (void) errbuf;
   res->nhits = 0; res->hits = NULL;
   if (!a->greedy) return hitacc_finish_gamma(a, a->cutoff, res);
-  HREC **p = (HREC **) malloc(sizeof(HREC *) * (a->n ? a->n : 1));
+  const size_t ncap = a->n > 0 ? (size_t) a->n : 1;
+  HREC **p = (HREC **) malloc(sizeof(HREC *) * ncap);
   for (int h = 0; h < a->n; h++) p[h] = &a->h[h];
   if (a->n > 1) qsort(p, a->n, sizeof(HREC *), hit_cmp);
   for (int i = 0; i < a->n; i++) {
@@ -238,7 +239,7 @@ static int hitacc_finish(HITACC *a, ORACLE_RESULT *res, char *errbuf)
       if (!(p[k]->stop < p[i]->start) && !(p[i]->stop < p[k]->start)) p[k]->removed = 1;
     }
   }
-  res->hits = (ORACLE_HIT *) malloc(sizeof(ORACLE_HIT) * (a->n ? a->n : 1));
+  res->hits = (ORACLE_HIT *) malloc(sizeof(ORACLE_HIT) * ncap);
   for (int i = 0; i < a->n; i++) {
     if (p[i]->removed) continue;
     ORACLE_HIT *o = &res->hits[res->nhits++];
