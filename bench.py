@@ -127,7 +127,9 @@ def cyk_1500(args, ia, synth, threads):
     frags = synth.homologs(ia.CM.read(path), max(1, n // 20000), seed=15)
     codes, _ = synth.plant(bg, frags, seed=16)
     dsq = ia.dsq_from_codes(codes)
-    ia.FastCYKScan(cm, 1, dsq, 1, min(n, 1600), 0.0, want_hits=True)   # first-use set-up, one short chunk
+    # first use (profile build, device buffers sized for this target), untimed, as the benchmark's
+    # warm-up steps are
+    ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
     t0 = time.perf_counter()
     r = ia.FastCYKScan(cm, 1, dsq, 1, n, 0.0, want_hits=True)
     wall = time.perf_counter() - t0

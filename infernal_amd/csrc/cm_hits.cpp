@@ -10,6 +10,7 @@
  *                          :400-420, :1097-1275, CloneHitMostly :877-915
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
@@ -224,6 +225,71 @@ static bool cand_before(const INFX_CAND &a, const INFX_CAND &b)
   return a.d < b.d;
 }
 
+/* The permutation that orders key[0..n-1] ascending, ties in index order (a
+ * stable LSD radix sort over 11-bit digits that carries the keys with the
+ * indices, so every pass reads sequentially; digits equal in every key are
+ * skipped).  O(n) per digit: the comparison sorts it replaces took most of
+ * the replay for the tens of millions of candidates a long target gives. */
+static void radix_order(std::vector<uint64_t> &key, std::vector<uint32_t> &perm)
+{
+  const size_t n = key.size();
+  perm.resize(n);
+  for (size_t i = 0; i < n; i++) perm[i] = (uint32_t) i;
+  uint64_t all_or = 0, all_and = ~0ull;
+  for (size_t i = 0; i < n; i++) { all_or |= key[i]; all_and &= key[i]; }
+  std::vector<uint64_t> tkey(n);
+  std::vector<uint32_t> tperm(n);
+  std::vector<size_t> cnt(2049);
+  for (int sh = 0; sh < 64; sh += 11) {
+    if ((((all_or ^ all_and) >> sh) & 0x7ffull) == 0) continue;   /* this digit is the same in every key */
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t i = 0; i < n; i++) cnt[((key[i] >> sh) & 0x7ff) + 1]++;
+    for (size_t b = 1; b <= 2048; b++) cnt[b] += cnt[b - 1];
+    for (size_t i = 0; i < n; i++) {
+      const size_t o = cnt[(key[i] >> sh) & 0x7ff]++;
+      tkey[o] = key[i];
+      tperm[o] = perm[i];
+    }
+    key.swap(tkey);
+    perm.swap(tperm);
+  }
+}
+
+/* Candidates in (j, d) order: radix-ordered by j (every j of a call lies in
+ * one 2^32 range above its smallest), then each j's run checked for
+ * ascending d and sorted where it is not (the device writes one row's cells
+ * in ascending d, so the runs come out sorted).  Small sets use std::sort.
+ * The (j, d) pairs of one call are distinct, so the order is the one a
+ * comparison sort gives. */
+static void sort_cands(INFX_CAND *cand, int64_t n)
+{
+  if (n < ((int64_t) 1 << 16) || n > (int64_t) UINT32_MAX) { std::sort(cand, cand + n, cand_before); return; }
+  int32_t jmin = INT32_MAX;
+  for (int64_t k = 0; k < n; k++) jmin = std::min(jmin, cand[k].j);
+  std::vector<uint64_t> key((size_t) n);
+  for (int64_t k = 0; k < n; k++) key[k] = (uint64_t) ((int64_t) cand[k].j - (int64_t) jmin);
+  std::vector<uint32_t> perm;
+  radix_order(key, perm);
+  std::vector<INFX_CAND> out((size_t) n);
+  for (int64_t k = 0; k < n; k++) out[k] = cand[perm[k]];
+  for (int64_t c = 0; c < n; ) {
+    int64_t e = c + 1;
+    while (e < n && out[e].j == out[c].j) e++;
+    if (!std::is_sorted(out.begin() + c, out.begin() + e, cand_before)) std::sort(out.begin() + c, out.begin() + e, cand_before);
+    c = e;
+  }
+  memcpy(cand, out.data(), sizeof(INFX_CAND) * (size_t) n);
+}
+
+/* order-preserving unsigned key of a non-NaN float (-0 and +0 equal) */
+static uint32_t float_key(float x)
+{
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if (u == 0x80000000u) u = 0;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 /* Sliding act vectors: act[jp % (W+1)] = residue counts over i0..i0+jp-1,
  * accumulated in the reference's order (src/cm_dpsearch.c:223-230). */
 struct ActRing {
@@ -260,7 +326,7 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
                                  int greedy, CM_TOPHITS *hitlist)
 {
   if (hitlist == NULL) return eslOK;
-  std::sort(cand, cand + ncand, cand_before);
+  sort_cands(cand, ncand);
   ActRing act(W);
   int status;
 
@@ -300,11 +366,26 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
       }
       c = e;
     }
-    std::sort(hs.begin(), hs.end(), [](const CHit &a, const CHit &b) {
+    auto hit_order = [](const CHit &a, const CHit &b) {
       if (a.score != b.score) return a.score > b.score;
       if (a.start != b.start) return a.start < b.start;
       return a.idx < b.idx;
-    });
+    };
+    if (hs.size() < ((size_t) 1 << 16) || hs.size() > (size_t) UINT32_MAX || j0 - i0 >= (int64_t) UINT32_MAX) {
+      std::sort(hs.begin(), hs.end(), hit_order);
+    } else {
+      /* the same order by one 64-bit key: score descending, then start
+       * ascending; hs is in creation (idx) order, and the radix sort is
+       * stable */
+      std::vector<uint64_t> key(hs.size());
+      for (size_t k = 0; k < hs.size(); k++)
+        key[k] = ((uint64_t) ~float_key(hs[k].score) << 32) | (uint64_t) (hs[k].start - i0);
+      std::vector<uint32_t> perm;
+      radix_order(key, perm);
+      std::vector<CHit> sorted(hs.size());
+      for (size_t k = 0; k < hs.size(); k++) sorted[k] = hs[perm[k]];
+      hs.swap(sorted);
+    }
     const int64_t L = j0 - i0 + 1;
     std::vector<uint64_t> used((size_t) (L + 64) / 64 + 1, 0);
     auto any_used = [&](int64_t lo, int64_t hi) {        /* positions lo..hi, 0-based in the call's range */

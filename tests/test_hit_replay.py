@@ -93,6 +93,75 @@ def test_greedy_replay_matches_reference_restatement(seed):
     assert got == want
 
 
+def reference_fast(js, ds, scs, rs, cutoff, i0, j0):
+    """The same restatement for large sets: candidates in (j, d) order as
+    arrays; overlap removal over a boolean array of used positions (a hit is
+    dropped if it shares a position with an earlier kept one, which is what
+    the quadratic scan decides)."""
+    hits = []
+    n = len(js)
+    k = 0
+    while k < n:
+        j = js[k]
+        mx = np.float32(-1e36)
+        while k < n and js[k] == j:
+            sc = scs[k]
+            if sc > mx and sc >= np.float32(cutoff) and sc > np.float32(-9.999e35):
+                hits.append((int(j - ds[k] + 1), int(j), sc, int(rs[k])))
+                mx = sc
+            k += 1
+    order = sorted(range(len(hits)), key=lambda q: (-hits[q][2], hits[q][0]))   # stable
+    used = np.zeros(j0 - i0 + 2, dtype=bool)
+    out = []
+    for q in order:
+        st, sp, sc, r = hits[q]
+        if used[st - i0:sp - i0 + 1].any():
+            continue
+        used[st - i0:sp - i0 + 1] = True
+        out.append((st, sp, float(sc), r))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_greedy_replay_large_set(seed):
+    """Candidate sets above the size where the replay orders candidates and
+    hits by radix sorts (65,536) instead of comparison sorts: the same hits,
+    in the same order, as the restatement; ties in score and start common,
+    candidates handed over in a random order."""
+    lib = ia.lib()
+    f = lib.infx_resolve_hits
+    f.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                  C.c_int, C.c_float, C.c_int, C.c_int, C.c_void_p]
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    rng = np.random.default_rng(100 + seed)
+    i0, j0, W = 5001, 405000, 60
+    n = 300000
+    js = rng.integers(i0, j0 + 1, n)
+    ds = np.minimum(rng.integers(1, W + 1, n), js - i0 + 1)
+    key = js.astype(np.int64) * 1024 + ds
+    _, first = np.unique(key, return_index=True)       # one candidate per (j, d)
+    js, ds = js[first], ds[first]
+    m = len(js)
+    assert m > 65536 * 3
+    scs = (rng.integers(-20, 60, m) / 2.0).astype(np.float32)
+    scs[rng.random(m) < 0.01] = np.float32(-0.0)        # -0 and +0 tie, as the comparison does
+    rs = rng.integers(0, 50, m).astype(np.int32)
+    perm = rng.permutation(m)
+    arr = np.zeros(m, dtype=[("j", "<i4"), ("d", "<i4"), ("sc", "<f4"), ("r", "<i4"), ("mode", "<i4")])
+    arr["j"], arr["d"], arr["sc"], arr["r"], arr["mode"] = js[perm], ds[perm], scs[perm], rs[perm], 3
+    th = lib.cm_tophits_Create()
+    eb = C.create_string_buffer(128)
+    cutoff = 0.0
+    st = f(C.cast(cm.ptr, C.c_void_p), eb, 0, 0, arr.ctypes.data, m, None, i0, j0, W, cutoff, 0, 1, C.cast(th, C.c_void_p))
+    assert st == 0, eb.value
+    got = [(h[0], h[1], h[4], h[2]) for h in ia._hits_list(th)]
+    lib.cm_tophits_Destroy(th)
+    o = np.lexsort((ds, js))
+    want = reference_fast(js[o], ds[o], scs[o], rs[o], cutoff, i0, j0)
+    assert len(got) > 1000
+    assert got == want
+
+
 def _allow(cm, pass_any, local, start, stop, i0, j0, mode, b):
     """cm_hit_AllowTruncation (src/cm_tophits.c:3048) from the begin state's
     node span, as the truncated kernel's filter evaluates it."""
