@@ -319,7 +319,10 @@ int         cm_tophits_CloneHitMostly(CM_TOPHITS *src_th, int h, CM_TOPHITS *des
  * functions of the same name; hits are appended to <hitlist> after greedy
  * overlap removal (or gamma traceback when cm->search_opts has
  * CM_SEARCH_CMNOTGREEDY).  *ret_vsc, when requested, is malloc'ed and owned
- * by the caller.
+ * by the caller.  Positions must stay below 2^31 - 1 (j0 <= 2147483646):
+ * the device's hit candidates carry 32-bit positions, and a larger j0 is
+ * refused with eslERANGE (cmsearch scans buffers of one sequence window,
+ * whose positions stay below ~10^5).
  */
 
 /* src/cm_dpsearch.c:90 FastCYKScan */

@@ -582,6 +582,9 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   if (qdbidx < 0 || qdbidx > 2)       return infx_fail(eslEINCOMPAT, errbuf, "%s, qdbidx is invalid", fname);
   if (i0 < 1)                         return infx_fail(eslEINCOMPAT, errbuf, "%s, i0 must be >= 1", fname);
   if (j0 - i0 + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
+  /* hit candidates carry 32-bit positions (cmsearch scans buffers of one
+   * sequence window, positions below ~10^5) */
+  if (j0 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, j0 = %lld: positions past %d do not fit the hit candidates", fname, (long long) j0, INT32_MAX - 1);
   int64_t L = 0, Lmax = 0;      /* residues scanned; the longest window */
   const uint8_t *pinned = infx_pinned(dsq, i0, j0);   /* residues already on the device (checked when pinned) */
   for (int w = 0; w < nwin; w++) {
@@ -1082,6 +1085,7 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
     Ltot += wj[w] - wi[w] + 1;
   }
   if (span_hi - span_lo + 1 > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, target range too long for one call", fname);
+  if (span_hi > (int64_t) INT32_MAX - 1) return infx_fail(eslERANGE, errbuf, "%s, j0 = %lld: positions past %d do not fit the hit candidates", fname, (long long) span_hi, INT32_MAX - 1);
   const uint8_t *pinned = infx_pinned(dsq, span_lo, span_hi);   /* residues already on the device (checked when pinned) */
   for (int64_t x = span_lo; !pinned && x <= span_hi; x++)
     if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);

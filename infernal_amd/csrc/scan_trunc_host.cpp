@@ -362,6 +362,8 @@ static int tr_scan_windows(int semi, const char *fname, CM_t *cm, char *errbuf, 
     if (wj0[w] < wi0[w]) return infx_fail(eslEINCOMPAT, errbuf, "%s, i0: %lld j0: %lldd\n", fname, (long long) wi0[w], (long long) wj0[w]);
     if (wi0[w] < 1) return infx_fail(eslEINCOMPAT, errbuf, "%s, i0 must be >= 1", fname);
     if (wj0[w] - wi0[w] + 1 > INT32_MAX / 2) return infx_fail(eslERANGE, errbuf, "%s, window too long", fname);
+    if (wj0[w] > (int64_t) INT32_MAX - 1)   /* hit candidates carry 32-bit positions */
+      return infx_fail(eslERANGE, errbuf, "%s, j0 = %lld: positions past %d do not fit the hit candidates", fname, (long long) wj0[w], INT32_MAX - 1);
     for (int64_t x = wi0[w]; x <= wj0[w]; x++)
       if (dsq[x] > 15) return infx_fail(eslEINVAL, errbuf, "%s, residue code %d at %lld is not a residue", fname, (int) dsq[x], (long long) x);
     maxlen = ESL_MAX(maxlen, wj0[w] - wi0[w] + 1);

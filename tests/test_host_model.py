@@ -175,6 +175,26 @@ def test_contract_errors():
     assert e.value.status == ia.eslEINVAL
 
 
+def test_positions_past_32_bits_are_refused():
+    """Hit candidates carry 32-bit positions: a call whose j0 passes
+    2^31 - 1 is refused with eslERANGE before the residues are read (the
+    residue pointer given here is never dereferenced), by the standard and
+    the truncated scanners alike.  cmsearch scans buffers of one sequence
+    window, so its positions stay far below that."""
+    import ctypes as C
+    from helpers import load_cm
+    lib = ia.lib()
+    eb = C.create_string_buffer(128)
+    i0, j0 = 2 ** 31 - 100, 2 ** 31 + 100
+    fake = C.c_void_p(4096)                     # never read: the range check comes first
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    st = lib.FastCYKScan(cm.ptr, eb, cm.c.smx, 1, fake, i0, j0, 0.0, None, 0, 0.0, None, None, None, None)
+    assert st == 16 and b"do not fit the hit candidates" in eb.value, eb.value   # eslERANGE
+    cmt = load_cm("tRNA.c.cm", local=True, trunc=True)
+    st = lib.RefTrCYKScan(cmt.ptr, eb, cmt.c.trsmx, 1, 2, fake, i0, j0, 0.0, None, 0, 0.0, None, None, None, None, None)
+    assert st == 16 and b"do not fit the hit candidates" in eb.value, eb.value
+
+
 def test_batch_contract_errors():
     """infx_FastScanBatch checks its windows and every model before any
     device work: model indices in range, each model's windows sorted and
