@@ -776,6 +776,12 @@ static int std_scan(int semi, const char *fname, CM_t *cm, char *errbuf, CM_SCAN
   int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(ar ? eslERANGE : eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+  /* Few slots (a short target, a few windows): workgroups small enough to
+   * reach every CU, since a workgroup runs on one CU and 16-wavefront
+   * workgroups would leave most of the chip idle.  Cells and hits do not
+   * depend on the shape. */
+  if (!infx_wpb_override())
+    while (wpb > 1 && (int64_t) ((nslots + wpb - 1) / wpb) < (int64_t) g_ncu) wpb >>= 1;
   if (ar && (ar->give(D->d_dsq, pinned ? 64 : (size_t) Lspan + 128) != eslOK || ar->give(D->d_scratch, slot_bytes * nslots) != eslOK ||
              ar->give(D->d_chunk, chunk_out_bytes) != eslOK || ar->give(D->d_ncand, 64) != eslOK ||
              ar->give(D->d_args, sizeof(ScanArgs)) != eslOK ||
@@ -1206,6 +1212,9 @@ extern "C" int infx_FastScanBatch(int semi, int nmodel, CM_t **cms, CM_SCAN_MX *
   int nslots = (int) ESL_MIN((int64_t) nchunks, (int64_t) (scratch_budget / slot_bytes));
   nslots = (int) ESL_MIN((int64_t) nslots, resident);
   if (nslots < 1) return infx_fail(eslEMEM, errbuf, "%s: one DP slot needs %zu bytes", fname, slot_bytes);
+  /* few slots: workgroups small enough to reach every CU (as in std_scan) */
+  if (!infx_wpb_override())
+    while (wpb > 1 && (int64_t) ((nslots + wpb - 1) / wpb) < (int64_t) g_ncu) wpb >>= 1;
   const int64_t Lspan = span_hi - span_lo + 1;
   int64_t cand_cap = ESL_MAX((int64_t) 1 << 16, ESL_MIN((int64_t) 4 * Ltot, ((int64_t) 8 << 30) / (int64_t) sizeof(Cand)));
   cand_cap = ESL_MAX(cand_cap, (int64_t) D->d_cand.cap / (int64_t) sizeof(Cand));
