@@ -462,6 +462,13 @@ int infx_debug_attach_trdev(CM_t *cm, CM_TR_SCAN_MX *trsmx, int nprof);
  * No counterpart in the reference: used to check the schedule on the host. */
 int infx_debug_tr_schedule(CM_t *cm, CM_TR_SCAN_MX *trsmx, int qdbidx, int semi, int pass_idx, int nw, int want_vsc,
                            int *out_sched, int sched_cap, int *out_items, int items_cap, int *ret_nlev, char *errbuf);
+/* The float truncated-begin penalties of an unconfigured (global) model, as
+ * cm_tr_penalties_Create (src/cm_trunc.c:51) builds them with its
+ * ignore_inserts flag set or not, into out[2][NTRPENALTY][M] (glocal, then
+ * local); the model is not changed.  cm_Configure uses ignore_inserts = 0.
+ * Lets a CPU test apply the reference's own check of the penalties,
+ * cm_tr_penalties_Validate (src/cm_trunc.c:498), which assumes the flag. */
+int infx_debug_tr_penalties(const CM_t *cm, int ignore_inserts, float *out, char *errbuf);
 /* Query-dependent bands and W from the model's transition probabilities
  * (local ends off, standard local begins on).  Fills qdbinfo->dmin1/dmax1
  * (beta1) and dmin2/dmax2 (beta2) for every state and *ret_W (beta_W).

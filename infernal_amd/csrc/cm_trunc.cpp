@@ -71,14 +71,30 @@ template <typename T> static T **alloc2(int r, int c, T val)
   return a;
 }
 
-extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
+static void tr_penalties_free(CM_TR_PENALTIES *trp)
+{
+  if (!trp) return;
+  CM_TR_PENALTIES *t = trp;
+  if (t->g_ptyAA)  { free(t->g_ptyAA[0]);  free(t->g_ptyAA); }
+  if (t->l_ptyAA)  { free(t->l_ptyAA[0]);  free(t->l_ptyAA); }
+  if (t->ig_ptyAA) { free(t->ig_ptyAA[0]); free(t->ig_ptyAA); }
+  if (t->il_ptyAA) { free(t->il_ptyAA[0]); free(t->il_ptyAA); }
+  free(t);
+}
+
+/* ignore_inserts: the reference's flag (src/cm_trunc.c:51): the whole
+ * probability of a node goes to its match or bifurcation state, none to
+ * the inserts before it.  cm_Configure builds the table without it. */
+static int tr_penalties_build(const CM_t *cm, int ignore_inserts, CM_TR_PENALTIES **ret_trp, char *errbuf)
 {
   int status;
+  *ret_trp = NULL;
   if (cm->emap == NULL) return infx_fail(eslEINVAL, errbuf, "truncation penalties need the emit map");
   const int M = cm->M;
   CM_TR_PENALTIES *trp = (CM_TR_PENALTIES *) calloc(1, sizeof(CM_TR_PENALTIES));
+  *ret_trp = trp;
   trp->M = M;
-  trp->ignored_inserts = 0;
+  trp->ignored_inserts = ignore_inserts ? 1 : 0;
   trp->g_ptyAA  = alloc2<float>(NTRPENALTY, M, (float) IMPOSSIBLE);
   trp->l_ptyAA  = alloc2<float>(NTRPENALTY, M, (float) IMPOSSIBLE);
   trp->ig_ptyAA = alloc2<int>(NTRPENALTY, M, -INFTY);
@@ -97,7 +113,7 @@ extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
         begin[cm->nodemap[nd]] = p;
   }
   std::vector<double> psi;
-  if ((status = expected_state_occupancy(cm, psi, errbuf)) != eslOK) { cm->trp = trp; return status; }
+  if ((status = expected_state_occupancy(cm, psi, errbuf)) != eslOK) return status;
 
   float g_5and3 = 2. / (cm->clen * (cm->clen + 1));
   float g_5or3  = 1. / cm->clen;
@@ -122,6 +138,10 @@ extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
       float i1_psi = (i1 == -1) ? 0. : psi[i1];
       float i2_psi = (i2 == -1) ? 0. : psi[i2];
       float summed_psi = m_psi + i1_psi + i2_psi;
+      if (ignore_inserts) {
+        i1_psi = i2_psi = 0.;
+        summed_psi = m_psi;
+      }
 
       trp->g_ptyAA[TRPENALTY_5P_AND_3P][m] = (m_psi / summed_psi) * g_5and3;
       if (i1 != -1) trp->g_ptyAA[TRPENALTY_5P_AND_3P][i1] = (i1_psi / summed_psi) * g_5and3;
@@ -162,10 +182,8 @@ extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
                          ((cm->sttype[v] == IL_st || cm->sttype[v] == IR_st) && !StateIsDetached(cm, v));
     if (!emitter_begin) continue;
     if (cm->stid[v] == MATP_IL && cm->ndtype[cm->ndidx[v] + 1] == END_nd) continue;
-    if (!NOT_IMPOSSIBLE(trp->g_ptyAA[TRPENALTY_5P_AND_3P][v])) {
-      cm->trp = trp;
+    if (!NOT_IMPOSSIBLE(trp->g_ptyAA[TRPENALTY_5P_AND_3P][v]))
       return infx_fail(eslEINVAL, errbuf, "couldn't create truncation penalties for the CM (state %d)", v);
-    }
     trp->ig_ptyAA[TRPENALTY_5P_AND_3P][v] = Prob2Score(trp->g_ptyAA[TRPENALTY_5P_AND_3P][v], 1.0);
     trp->g_ptyAA[TRPENALTY_5P_AND_3P][v]  = sreLOG2(trp->g_ptyAA[TRPENALTY_5P_AND_3P][v]);
     if (NOT_IMPOSSIBLE(trp->g_ptyAA[TRPENALTY_5P_ONLY][v])) {
@@ -183,6 +201,33 @@ extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
     trp->l_ptyAA[TRPENALTY_5P_ONLY][v]    = sreLOG2(trp->l_ptyAA[TRPENALTY_5P_ONLY][v]);
     trp->l_ptyAA[TRPENALTY_3P_ONLY][v]    = sreLOG2(trp->l_ptyAA[TRPENALTY_3P_ONLY][v]);
   }
-  cm->trp = trp;
   return eslOK;
+}
+
+/* cm->trp for cm_Configure (on a failure the partial table is attached, and
+ * released with the model) */
+extern "C" int infx_tr_penalties_create(CM_t *cm, char *errbuf)
+{
+  CM_TR_PENALTIES *trp = NULL;
+  const int status = tr_penalties_build(cm, 0, &trp, errbuf);
+  cm->trp = trp;
+  return status;
+}
+
+/* The float penalty tables of an unconfigured (global) model, built with or
+ * without the inserts' shares, into out[2][NTRPENALTY][M]: glocal, then
+ * local; the model is not changed. */
+extern "C" int infx_debug_tr_penalties(const CM_t *cm, int ignore_inserts, float *out, char *errbuf)
+{
+  CM_TR_PENALTIES *trp = NULL;
+  const int status = tr_penalties_build(cm, ignore_inserts, &trp, errbuf);
+  if (status == eslOK) {
+    const size_t M = (size_t) cm->M;
+    for (int i = 0; i < NTRPENALTY; i++) {
+      memcpy(out + (size_t) i * M, trp->g_ptyAA[i], M * sizeof(float));
+      memcpy(out + (size_t) (NTRPENALTY + i) * M, trp->l_ptyAA[i], M * sizeof(float));
+    }
+  }
+  tr_penalties_free(trp);
+  return status;
 }
