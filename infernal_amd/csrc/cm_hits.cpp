@@ -338,8 +338,12 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
      * same cm, sequence and strand -- and kept unless they overlap a kept hit
      * (a bitmap of kept positions).  Only the survivors become CM_HITs, built
      * and cloned exactly as cm_tophits_CloneHitMostly would. */
-    struct CHit { int64_t start, stop; float score; int root; int mode; int64_t idx; };
+    /* positions as offsets from i0 (a call spans fewer than 2^31 residues) */
+    struct CHit { int32_t lo, hi; float score; int32_t root; int32_t mode; uint32_t idx; };
+    if (j0 - i0 >= (int64_t) INT32_MAX || ncand > (int64_t) UINT32_MAX)
+      return infx_fail(eslERANGE, errbuf, "hit replay: %lld positions, %lld candidates", (long long) (j0 - i0 + 1), (long long) ncand);
     std::vector<CHit> hs;
+    hs.reserve((size_t) ncand);
     int64_t c = 0;
     while (c < ncand) {
       int64_t j = cand[c].j, e = c;
@@ -359,32 +363,31 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
             do_report = (hit_sc > max_sc_reported && hit_sc >= cutoff) ? 1 : 0;
           }
           if (do_report) {
-            hs.push_back(CHit{i, j, hit_sc, cand[k].r, cand[k].mode, (int64_t) hs.size()});
+            hs.push_back(CHit{(int32_t) (i - i0), (int32_t) (j - i0), hit_sc, cand[k].r, cand[k].mode, (uint32_t) hs.size()});
             max_sc_reported = hit_sc;
           }
         }
       }
       c = e;
     }
-    auto hit_order = [](const CHit &a, const CHit &b) {
-      if (a.score != b.score) return a.score > b.score;
-      if (a.start != b.start) return a.start < b.start;
-      return a.idx < b.idx;
-    };
-    if (hs.size() < ((size_t) 1 << 16) || hs.size() > (size_t) UINT32_MAX || j0 - i0 >= (int64_t) UINT32_MAX) {
-      std::sort(hs.begin(), hs.end(), hit_order);
+    /* the order of the hits: score descending, start ascending, creation */
+    std::vector<uint32_t> order;
+    if (hs.size() < ((size_t) 1 << 16)) {
+      order.resize(hs.size());
+      for (size_t k = 0; k < hs.size(); k++) order[k] = (uint32_t) k;
+      std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        const CHit &a = hs[x], &b = hs[y];
+        if (a.score != b.score) return a.score > b.score;
+        if (a.lo != b.lo) return a.lo < b.lo;
+        return a.idx < b.idx;
+      });
     } else {
       /* the same order by one 64-bit key: score descending, then start
-       * ascending; hs is in creation (idx) order, and the radix sort is
-       * stable */
+       * ascending; hs is in creation order, and the radix sort is stable */
       std::vector<uint64_t> key(hs.size());
       for (size_t k = 0; k < hs.size(); k++)
-        key[k] = ((uint64_t) ~float_key(hs[k].score) << 32) | (uint64_t) (hs[k].start - i0);
-      std::vector<uint32_t> perm;
-      radix_order(key, perm);
-      std::vector<CHit> sorted(hs.size());
-      for (size_t k = 0; k < hs.size(); k++) sorted[k] = hs[perm[k]];
-      hs.swap(sorted);
+        key[k] = ((uint64_t) ~float_key(hs[k].score) << 32) | (uint64_t) (uint32_t) hs[k].lo;
+      radix_order(key, order);
     }
     const int64_t L = j0 - i0 + 1;
     std::vector<uint64_t> used((size_t) (L + 64) / 64 + 1, 0);
@@ -396,13 +399,14 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
       return false;
     };
     CM_TOPHITS *tmp = cm_tophits_Create();
-    for (const CHit &h : hs) {
-      const int64_t lo = h.start - i0, hi = h.stop - i0;
+    for (uint32_t q : order) {
+      const CHit &h = hs[q];
+      const int64_t lo = h.lo, hi = h.hi;
       if (any_used(lo, hi)) continue;
       for (int64_t x = lo; x <= hi; x++) used[x >> 6] |= 1ull << (x & 63);
       CM_HIT *hit;
       if ((status = cm_tophits_CreateNextHit(tmp, &hit)) != eslOK) { cm_tophits_Destroy(tmp); return infx_fail(status, errbuf, "out of memory"); }
-      hit->start = h.start; hit->stop = h.stop; hit->root = h.root;
+      hit->start = i0 + h.lo; hit->stop = i0 + h.hi; hit->root = h.root;
       hit->mode = (char) h.mode; hit->score = h.score; hit->hmmonly = 0; hit->srcL = j0;
     }
     for (uint64_t h = 0; h < tmp->N; h++) tmp->hit[h] = tmp->unsrt + h;   /* already in sorted order */
