@@ -402,7 +402,7 @@ def ScanBatch(algo, cms, qdbidx, dsq, windows, cutoffs, greedy=True):
 def TrScanWindows(algo, cm, qdbidx, pass_idx, dsq, windows, cutoff, do_null3=False, env_cutoff=0.0, want_hits=True,
                   greedy=True):
     """infx_TrScanWindows: one truncated scan per window [(i, j), ...] of one
-    target, all in one launch (one wavefront per window); algo "cyk"
+    target, all in one launch (one workgroup per window); algo "cyk"
     (RefTrCYKScan) or "iinside" (RefITrInsideScan).  Per window: best score,
     its mode, envelope and (want_hits) hit list, as one call per window."""
     inside = {"cyk": 0, "iinside": 1}[algo]

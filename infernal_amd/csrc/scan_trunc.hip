@@ -26,7 +26,7 @@
  * taken depends on every earlier j of the same parity (src/cm_dpsearch_trunc.c
  * :538-605).  Truncated scans run on short terminal windows (the pipeline's
  * truncated passes use windows of length maxW), and many windows are batched
- * into one launch, one wavefront each.
+ * into one launch, one workgroup each.
  *
  * IL states are the only same-j chains (J and L modes read the state's own
  * d-1 cell); they are solved by the exact fixed-point sweep described in

@@ -2,7 +2,7 @@
  * RefTrCYKScan (src/cm_dpsearch_trunc.c:79) and RefITrInsideScan (:741):
  * the device profile of a configured CM, the windowed launch and the C-ABI
  * entry points.  infx_TrScanWindows runs many windows of one model in a single
- * launch (one wavefront per window).
+ * launch (one workgroup per window).
  */
 #include <hip/hip_runtime.h>
 #include <limits.h>

@@ -195,6 +195,44 @@ def test_positions_past_32_bits_are_refused():
     assert st == 16 and b"do not fit the hit candidates" in eb.value, eb.value
 
 
+def test_windowed_entry_points_contract_errors():
+    """infx_FastScanWindows and infx_TrScanWindows check their windows
+    before any device work: sorted and disjoint (standard), j0 >= i0 and
+    i0 >= 1 (truncated), residue codes, the pass index; without a GPU a
+    valid call reports eslENODEVICE."""
+    from helpers import load_cm
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    dsq = ia.dsq_from_codes(np.zeros(600, dtype=np.uint8))
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanWindows("cyk", cm, 1, dsq, [(1, 100), (50, 150)], 0.0)
+    assert e.value.status == ia.eslEINCOMPAT and "sorted and disjoint" in str(e.value)
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanWindows("cyk", cm, 1, dsq, [(200, 300), (1, 100)], 0.0)
+    assert e.value.status == ia.eslEINCOMPAT
+    bad = dsq.copy()
+    bad[250] = 17
+    with pytest.raises(ia.ScanError) as e:
+        ia.ScanWindows("finside", cm, 2, bad, [(1, 100), (200, 300)], 0.0)
+    assert e.value.status == ia.eslEINVAL
+    cmt = load_cm("tRNA.c.cm", local=True, trunc=True)
+    with pytest.raises(ia.ScanError) as e:
+        ia.TrScanWindows("cyk", cmt, 1, 2, dsq, [(100, 50)], 0.0)
+    assert e.value.status == ia.eslEINCOMPAT
+    with pytest.raises(ia.ScanError) as e:
+        ia.TrScanWindows("cyk", cmt, 1, 99, dsq, [(1, 100)], 0.0)
+    assert e.value.status in (ia.eslEINVAL, ia.eslEINCOMPAT)
+    with pytest.raises(ia.ScanError) as e:
+        ia.TrScanWindows("iinside", cmt, 2, 2, bad, [(200, 300)], 0.0)
+    assert e.value.status == ia.eslEINVAL
+    if not ia.gpu_available():
+        with pytest.raises(ia.ScanError) as e:
+            ia.ScanWindows("cyk", cm, 1, dsq, [(1, 100), (200, 300)], 0.0)
+        assert e.value.status == ia.eslENODEVICE
+        with pytest.raises(ia.ScanError) as e:
+            ia.TrScanWindows("cyk", cmt, 1, 2, dsq, [(1, 100), (200, 300)], 0.0)
+        assert e.value.status == ia.eslENODEVICE
+
+
 def test_batch_contract_errors():
     """infx_FastScanBatch checks its windows and every model before any
     device work: model indices in range, each model's windows sorted and
