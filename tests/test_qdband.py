@@ -144,3 +144,26 @@ def test_bands_match_shipped_files(name):
     worst = max((abs(t[5] - 1.0) for t in ties), default=0.0)
     print(name, "W", W, "Z", Z, "columns off by one: %d of %d, tail mass at most %.2g from beta" %
           (len(ties), 4 * int(keep.sum()), worst))
+
+
+@pytest.mark.parametrize("name", MODELS)
+def test_bands_independent_of_Z(name):
+    """The reference's bandcyk-truncation-test (src/bandcyk-truncation-test.c,
+    test 1): bands and W computed with two lengths Z are identical, since
+    the engine refuses a Z whose truncated tail could move them (eslERANGE,
+    checked here on a Z far too short)."""
+    cm = ia.CM.read(os.path.join(CMDIR, name))
+    bands1, W1, Z1, _ = engine_bands(cm, cm.c.beta_W)
+    lib = ia.lib()
+    g = lib.infx_BandCalculationEngine
+    g.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int), C.c_void_p]
+    q = C.cast(cm.c.qdbinfo, C.c_void_p)
+    W2 = C.c_int()
+    assert g(C.cast(cm.ptr, C.c_void_p), 3 * Z1, q, cm.c.beta_W, C.byref(W2), None) == 0
+    bands2 = qdb_bands(cm)
+    assert W2.value == W1
+    keep = cm.sttype() != ia.E_st
+    for a, b in zip(bands1, bands2):
+        assert np.array_equal(a[keep], b[keep])
+    st = g(C.cast(cm.ptr, C.c_void_p), max(4, cm.clen // 4), q, cm.c.beta_W, C.byref(W2), None)
+    assert st == 16          # eslERANGE: Z too short
