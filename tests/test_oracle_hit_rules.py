@@ -172,6 +172,9 @@ def test_allow_truncation_rule(name, local):
                     got = lib.oracle_allow_truncation(C.cast(cm.ptr, C.c_void_p), p, start, stop, i0, j0, mode, b)
                     want = admissible(local, c.clen, p, start, stop, i0, j0, mode, first[nd], last[nd])
                     assert bool(got) == want, (b, nd, mode, p, start, stop, first[nd], last[nd])
+                    # the library's own function (what the GPU scanners' replay applies)
+                    got_lib = ia.lib().cm_hit_AllowTruncation(cm.ptr, p, start, stop, i0, j0, bytes([mode]), b)
+                    assert bool(got_lib) == want, ("library", b, nd, mode, p, start, stop)
 
 
 @pytest.mark.parametrize("local", [True, False], ids=["local", "glocal"])
