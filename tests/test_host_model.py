@@ -119,6 +119,30 @@ def test_localize_closed_forms():
     assert cl.flags & ia.CMH_LOCAL_BEGIN and cl.flags & ia.CMH_LOCAL_END
 
 
+def test_begin_end_scores_closed_forms():
+    """CMLogoddsify's begin/end scores (src/cm.c:868-880): beginsc = log2
+    begin, endsc = log2 end (IMPOSSIBLE where the probability is 0), their
+    integer twins Prob2Score(p, 1) = floor(0.5 + 1000 log2 p) (-INFTY at 0),
+    and iel_selfsc = Prob2Score(2^el_selfsc, 1), on the localised model whose
+    probabilities test_localize_closed_forms pins."""
+    import math
+    for name in ("Vault.c.cm", "Cobalamin.c.cm"):
+        cm = ia.CM.read(os.path.join(CMDIR, name)).configure(local=True)
+        c = cm.c
+        n_on = 0
+        for v in range(c.M):
+            for prob, sc, isc in ((c.begin[v], c.beginsc[v], c.ibeginsc[v]), (c.end[v], c.endsc[v], c.iendsc[v])):
+                if prob == 0.0:
+                    assert sc < -9e35 and isc <= -987654321, (name, v)
+                    continue
+                lg = math.log(prob) * 1.44269504
+                assert abs(sc - lg) <= 1e-6 * max(1.0, abs(lg)), (name, v, sc, lg)
+                assert isc == math.floor(0.5 + 1000.0 * lg), (name, v, isc, lg)
+                n_on += 1
+        assert n_on > 50
+        assert c.iel_selfsc == math.floor(0.5 + 1000.0 * math.log(2.0 ** c.el_selfsc) * 1.44269504)
+
+
 def test_logsum_tables_closed_form():
     """FLogsum table entry i = log2(1 + 2^(-i/1000)); ILogsum table its x1000
     rounding (src/logsum.c:74-148).  The reference evaluates these with the
