@@ -40,6 +40,7 @@ windows, with E-values; truncated passes at the sequence ends.
 
 Not covered: the HMM filter stages and hit alignment.
 """
+import bisect
 import math
 
 import numpy as np
@@ -192,14 +193,20 @@ def remove_overlaps(hits):
     """Overlap removal over the whole search (cm_tophits_SortForOverlapRemoval
     + remove_or_mark_overlaps_one_seq_memeff, src/cm_tophits.c:220-245,
     1097-1150): per strand, by score descending then start ascending; a hit
-    sharing a residue with a kept hit is dropped."""
+    sharing a residue with a kept hit is dropped.  The kept intervals are
+    disjoint, so they are held sorted by their left end and a hit is checked
+    against the one kept interval that starts last at or before its right
+    end (O(log n) per hit)."""
     out = []
     for strand in sorted({h["strand"] for h in hits}):
         hs = sorted((h for h in hits if h["strand"] == strand), key=lambda h: (-h["score"], h["start"]))
-        kept = []
+        los, his = [], []
         for h in hs:
             lo, hi = min(h["start"], h["stop"]), max(h["start"], h["stop"])
-            if all(hi < min(k["start"], k["stop"]) or lo > max(k["start"], k["stop"]) for k in kept):
-                kept.append(h)
-        out += kept
+            k = bisect.bisect_right(los, hi)
+            if k and his[k - 1] >= lo:
+                continue
+            los.insert(k, lo)
+            his.insert(k, hi)
+            out.append(h)
     return out

@@ -223,3 +223,30 @@ def test_search_overlap_removal_best_first_per_strand():
     kept = pl.remove_overlaps(hs)
     got = sorted((h["strand"], h["start"], h["stop"]) for h in kept)
     assert got == [("+", 40, 90), ("+", 91, 120), ("+", 200, 150), ("-", 60, 20)]
+
+
+def test_search_overlap_removal_matches_quadratic_rule():
+    """The pipeline's overlap removal on random hits with ties, both strands,
+    against the quadratic scan it restates (a hit is dropped if it shares a
+    residue with any hit kept before it, in score-then-start order)."""
+    from infernal_amd import pipeline as pl
+    rng = np.random.default_rng(77)
+    hs = []
+    for _ in range(3000):
+        a = int(rng.integers(1, 20000))
+        b = a + int(rng.integers(0, 300))
+        strand = "+" if rng.random() < 0.5 else "-"
+        start, stop = (a, b) if strand == "+" else (b, a)
+        hs.append({"start": start, "stop": stop, "strand": strand, "score": float(rng.integers(0, 40)) / 4.0})
+    got = pl.remove_overlaps(hs)
+    want = []
+    for strand in ("+", "-"):
+        cand = sorted((h for h in hs if h["strand"] == strand), key=lambda h: (-h["score"], h["start"]))
+        kept = []
+        for h in cand:
+            lo, hi = min(h["start"], h["stop"]), max(h["start"], h["stop"])
+            if all(hi < min(k["start"], k["stop"]) or lo > max(k["start"], k["stop"]) for k in kept):
+                kept.append(h)
+        want += kept
+    assert [id(h) for h in got] == [id(h) for h in want]
+    assert len(got) > 50
