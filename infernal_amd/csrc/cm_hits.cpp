@@ -326,6 +326,12 @@ extern "C" int infx_resolve_hits(CM_t *cm, char *errbuf, int pass_idx, int trunc
                                  int greedy, CM_TOPHITS *hitlist)
 {
   if (hitlist == NULL) return eslOK;
+  /* every candidate lies in the call's range (a cell the scan computed); a
+   * set that breaks this is refused rather than indexed out of bounds */
+  for (int64_t k = 0; k < ncand; k++)
+    if (cand[k].j < i0 || cand[k].j > j0 || cand[k].d < 1 || (int64_t) cand[k].j - cand[k].d + 1 < i0)
+      return infx_fail(eslEINVAL, errbuf, "hit replay: candidate (j %d, d %d) outside %lld..%lld", cand[k].j, cand[k].d,
+                       (long long) i0, (long long) j0);
   sort_cands(cand, ncand);
   ActRing act(W);
   int status;

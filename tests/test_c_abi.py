@@ -14,6 +14,7 @@ Infernal's cm_pipeline.c would be.
 """
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -52,6 +53,51 @@ def test_truncated_matrix_teardown_under_asan():
         r = subprocess.run([exe, "teardown", os.path.join(CMDIR, name)], capture_output=True, text=True, env=env, timeout=300)
         assert r.returncode == 0 and "teardown ok" in r.stdout, r.stdout + r.stderr[-3000:]
         assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+
+
+_REPLAY_DRIVER = r"""
+import ctypes as C, sys
+import numpy as np
+lib = C.CDLL(sys.argv[1])
+class CAND(C.Structure):
+    _fields_ = [("j", C.c_int32), ("d", C.c_int32), ("sc", C.c_float), ("r", C.c_int32), ("mode", C.c_int32)]
+lib.cm_tophits_Create.restype = C.c_void_p
+lib.cm_tophits_Destroy.argtypes = [C.c_void_p]
+f = lib.infx_resolve_hits
+f.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+              C.c_int, C.c_float, C.c_int, C.c_int, C.c_void_p]
+rng = np.random.default_rng(5)
+i0, j0, W = 1000001, 1400000, 300
+js = rng.integers(i0, j0 + 1, 400000)
+ds = np.minimum(rng.integers(1, W + 1, len(js)), js - i0 + 1)
+_, first = np.unique(js.astype(np.int64) * 1024 + ds, return_index=True)
+js, ds = js[first], ds[first]
+for greedy in (1, 0):
+    arr = (CAND * len(js))()
+    a = np.frombuffer(arr, dtype=[("j", "<i4"), ("d", "<i4"), ("sc", "<f4"), ("r", "<i4"), ("mode", "<i4")])
+    a["j"], a["d"], a["sc"], a["r"], a["mode"] = js, ds, (rng.integers(-40, 80, len(js)) / 2.0), 0, 3
+    th = lib.cm_tophits_Create()
+    eb = C.create_string_buffer(128)
+    st = f(None, eb, 0, 0, C.cast(arr, C.c_void_p), len(js), None, i0, j0, W, 0.0, 0, greedy, th)
+    assert st == 0, eb.value
+    lib.cm_tophits_Destroy(th)
+print("replay ok", len(js))
+"""
+
+
+def test_hit_replay_under_asan():
+    """The host replay (radix orderings, compact hit records, bitmap pass,
+    gamma path) on a set above the radix threshold, in the library's host
+    code rebuilt with AddressSanitizer and loaded into a Python process with
+    the ASan runtime preloaded."""
+    subprocess.run(["make", "-C", os.path.join(ROOT, "infernal_amd", "csrc")], check=True, capture_output=True)
+    subprocess.run(["make", "-C", CDIR, "asan_lib"], check=True, capture_output=True)
+    lib = os.path.join(CDIR, "_asan", "libinfernal_gpu_asan.so")
+    asan_rt = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1", LD_PRELOAD=asan_rt)
+    r = subprocess.run([sys.executable, "-c", _REPLAY_DRIVER, lib], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and "replay ok" in r.stdout, r.stdout + r.stderr[-3000:]
+    assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
 
 
 def test_dlopen_binding_beside_same_named_host_functions():

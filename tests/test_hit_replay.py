@@ -242,3 +242,26 @@ def test_truncated_block_record_filter_keeps_every_reported_hit(local, pass_idx)
     assert len(kept) < len(cells) // 4
     assert len(everything) > 0
     assert filtered == everything
+
+
+def test_replay_refuses_candidates_outside_the_range():
+    """A candidate whose cell lies outside the call's range (start before i0,
+    end past j0, d < 1) is refused with eslEINVAL instead of being indexed."""
+    lib = ia.lib()
+    f = lib.infx_resolve_hits
+    f.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                  C.c_int, C.c_float, C.c_int, C.c_int, C.c_void_p]
+    cm = ia.CM.read(os.path.join(CMDIR, "tRNA.c.cm")).configure(local=True)
+    i0, j0, W = 101, 400, 60
+    for j, d in ((120, 25), (401, 3), (200, 0), (150, 40)):
+        arr = (CAND * 2)()
+        arr[0].j, arr[0].d, arr[0].sc, arr[0].r, arr[0].mode = 300, 10, 5.0, 0, 3
+        arr[1].j, arr[1].d, arr[1].sc, arr[1].r, arr[1].mode = j, d, 5.0, 0, 3
+        th = lib.cm_tophits_Create()
+        eb = C.create_string_buffer(128)
+        st = f(C.cast(cm.ptr, C.c_void_p), eb, 0, 0, C.cast(arr, C.c_void_p), 2, None, i0, j0, W, 0.0, 0, 1, C.cast(th, C.c_void_p))
+        lib.cm_tophits_Destroy(th)
+        ok = (j - d + 1 >= i0) and j <= j0 and d >= 1
+        assert (st == 0) == ok, (j, d, st, eb.value)
+        if not ok:
+            assert st == ia.eslEINVAL and b"outside" in eb.value
